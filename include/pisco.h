@@ -1,0 +1,213 @@
+/*
+ * pisco.h -- C-ABI of the MI355X (gfx950) pixel-space beam convolution.
+ *
+ * Drop-in boundary for PISCO's GPUConvolver path.  Every entry point takes
+ * plain pointers and sizes and returns an int status (PISCO_OK == 0); on
+ * failure pisco_last_error() describes the problem.  Nothing here calls
+ * exit(): where the reference printed and exited (CUDA_ERROR_CHECK,
+ * src/GpuConvolver/gpuconvolver.cu:8-32; launchers, cudaConv2.cu:262-268) or
+ * threw std::invalid_argument (gpuconvolver.cu:74-78, convolver.cpp:66-70),
+ * these functions return an error code.
+ *
+ * Conventions shared with the reference:
+ *   - sky maps are RING-ordered HEALPix maps of 12*nside^2 float pixels
+ *     (Sky, src/Sky/sky.hpp:15-20);
+ *   - beams are the first npix RING pixels of a HEALPix map of NSIDE
+ *     nside_beam centred on the north pole, four maps (I,Q,U,V response) per
+ *     detector, detectors a and b (PolBeam::aBeams/bBeams,
+ *     src/Polbeam/polbeam.hpp:51-52); the disc radius is
+ *     rhoMax = theta(pixel npix) (polbeam.cpp:35-36);
+ *   - pointing is float ra, dec, pa in radians with pa counter-clockwise
+ *     (Scan, src/Scan/scan.hpp:14-18); the kernels use psi = -pa
+ *     (convolver.cpp:49-51, gpuconvolver.cu:375-377);
+ *   - TOD follows the CPU Convolver (convolver.cpp:157-164): the V maps are
+ *     not used.
+ * Departures (deliberate): every sample is computed (the reference drops the
+ * nsamples % ptgPerConv tail, gpuconvolver.cu:386), and buffers are
+ * re-allocated when an update grows them (gpuconvolver.cu:279-290,354-367).
+ *
+ * Host-buffer entry points copy their inputs; callers may free them after the
+ * call returns.  pisco_dev_* entry points take device pointers and a
+ * hipStream_t (NULL = the context's own stream) and are asynchronous with
+ * respect to the host.
+ */
+#ifndef PISCO_H
+#define PISCO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PISCO_OK 0
+#define PISCO_ERR_INVALID 1      /* bad argument or configuration */
+#define PISCO_ERR_STATE 2        /* call order (update_* before exec) */
+#define PISCO_ERR_MEMORY 3       /* allocation failed or over maxMemUsage */
+#define PISCO_ERR_DEVICE 4       /* HIP runtime error */
+
+/* Mirrors CUDACONV::RunConfig (src/GpuConvolver/cuda/cudaConv2.h:13-26).
+ * maxMemUsage is checked against the reference's own estimate
+ * (gpuconvolver.cu:41-64); ptgPerConv is the scan chunk of one launch;
+ * gridSize / blockSize / pixelsPerDisc are accepted and, as in the
+ * reference, not used to size anything (the kernels size their own
+ * work from the exact discs). */
+typedef struct pisco_run_config {
+    int nStreams;
+    size_t maxMemUsage;
+    int deviceId;
+    int gridSizeX;
+    int gridSizeY;
+    int blockSizeX;
+    int blockSizeY;
+    int ptgPerConv;
+    int pixelsPerDisc;
+} pisco_run_config;
+
+typedef struct pisco_ctx pisco_ctx;
+
+/* Text of the last error of the calling thread ("" if none). */
+const char *pisco_last_error(void);
+
+/* Library / device information: writes up to len bytes. */
+int pisco_version(char *buf, int len);
+
+/* GPUConvolver::GPUConvolver(RunConfig) + configure_execution()
+ * (gpuconvolver.cu:34-87): validates the configuration, selects
+ * cfg->deviceId and creates the context's stream. */
+int pisco_create(const pisco_run_config *cfg, pisco_ctx **out);
+
+/* GPUConvolver::~GPUConvolver (gpuconvolver.cu:172-221). NULL is ignored. */
+void pisco_destroy(pisco_ctx *ctx);
+
+/* GPUConvolver::update_sky(Sky*) (gpuconvolver.cu:223-269).  V may be NULL. */
+int pisco_update_sky(pisco_ctx *ctx, int nside, const float *I, const float *Q,
+                     const float *U, const float *V);
+
+/* GPUConvolver::update_beam(PolBeam*) (gpuconvolver.cu:271-345): a[k], b[k]
+ * are PolBeam::aBeams[k], bBeams[k] (k = 0..3, npix floats each; a[3]/b[3]
+ * may be NULL). */
+int pisco_update_beam(pisco_ctx *ctx, int nside_beam, long npix,
+                      const float *const a[4], const float *const b[4]);
+
+/* GPUConvolver::update_scan(Scan*) (gpuconvolver.cu:347-380). */
+int pisco_update_scan(pisco_ctx *ctx, long nsamples, const float *ra,
+                      const float *dec, const float *pa);
+
+/* GPUConvolver::exec_convolution(data_a, data_b) (gpuconvolver.cu:382-505)
+ * with the polFlag of Convolver::exec_convolution (convolver.cpp:22-72):
+ * 'a' writes tod_a, 'b' writes tod_b, 'p' writes both; the GPUConvolver
+ * form (no flag) is polflag 'p'. */
+int pisco_exec_convolution(pisco_ctx *ctx, char polflag, float *tod_a,
+                           float *tod_b);
+
+/* Adjoint with respect to the sky (no counterpart in the reference: the
+ * transpose of convolver.cpp:157-164): gI,gQ,gU (12*nside^2 floats each)
+ * = A^T (g_a, g_b).  g_a or g_b may be NULL (treated as zero). */
+int pisco_exec_adjoint_map(pisco_ctx *ctx, const float *g_a, const float *g_b,
+                           float *gI, float *gQ, float *gU);
+
+/* Adjoint with respect to the beam maps (no counterpart in the reference):
+ * ga[k], gb[k] receive d<g,TOD>/d aBeams[k], bBeams[k] (npix floats each;
+ * k = 3 is zero because the forward does not read V).  Entries may be NULL. */
+int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
+                            float *const ga[4], float *const gb[4]);
+
+/* ---- device-resident entry points (device pointers, asynchronous) ------- */
+
+/* Packs eight float beam maps (a0..a3, b0..b3; a3/b3 may be NULL) into the
+ * kernels' [npix][8] layout beam8 (8*npix floats, device). */
+int pisco_dev_pack_beam(pisco_ctx *ctx, void *stream, long npix,
+                        const float *const maps[8], float *beam8);
+
+/* Forward: tod_a/tod_b (either may be NULL) for nsamples samples. */
+int pisco_dev_forward(pisco_ctx *ctx, void *stream, int nside_sky,
+                      const float *I, const float *Q, const float *U,
+                      int nside_beam, long npix_beam, const float *beam8,
+                      long nsamples, const float *ra, const float *dec,
+                      const float *pa, float *tod_a, float *tod_b);
+
+/* Adjoint w.r.t. the sky: gI,gQ,gU (device, 12*nside_sky^2 floats each). */
+int pisco_dev_adjoint_map(pisco_ctx *ctx, void *stream, int nside_sky,
+                          int nside_beam, long npix_beam, const float *beam8,
+                          long nsamples, const float *ra, const float *dec,
+                          const float *pa, const float *g_a, const float *g_b,
+                          float *gI, float *gQ, float *gU);
+
+/* Adjoint w.r.t. the beams: gbeam8 (device, [npix_beam][8] floats, same
+ * layout as beam8). */
+int pisco_dev_adjoint_beam(pisco_ctx *ctx, void *stream, int nside_sky,
+                           const float *I, const float *Q, const float *U,
+                           int nside_beam, long npix_beam, long nsamples,
+                           const float *ra, const float *dec, const float *pa,
+                           const float *g_a, const float *g_b, float *gbeam8);
+
+/* ---- HEALPix / matching primitives on the device (for verification) ---- */
+
+/* Healpix_Base::pix2ang for n pixels (device arrays). */
+int pisco_dev_pix2ang(pisco_ctx *ctx, void *stream, int nside, long n,
+                      const int *pix, double *theta, double *phi);
+/* Healpix_Base::ang2pix(pointing) for n angles (device arrays). */
+int pisco_dev_ang2pix(pisco_ctx *ctx, void *stream, int nside, long n,
+                      const double *theta, const double *phi, int *pix);
+/* Healpix_Base::get_interpol: pix[4n], wgt[4n] (device arrays). */
+int pisco_dev_get_interpol(pisco_ctx *ctx, void *stream, int nside, long n,
+                           const double *theta, const double *phi, int *pix,
+                           double *wgt);
+/* Non-inclusive query_disc around (pi/2 - dec, ra) of n pointings: with
+ * pixels == NULL writes counts[n]; otherwise writes the pixel lists at
+ * offsets[i] (device arrays). */
+int pisco_dev_query_disc(pisco_ctx *ctx, void *stream, int nside, long n,
+                         const float *ra, const float *dec, double radius,
+                         const long *offsets, int *counts, int *pixels);
+/* Sky pixel -> (beam pixel, chi) for the pixel lists of n pointings
+ * (what kernel 1 of the reference computes, cudaConv2.cu:59-128). */
+int pisco_dev_match(pisco_ctx *ctx, void *stream, int nside_sky,
+                    int nside_beam, long n, const float *ra, const float *dec,
+                    const float *pa, const long *offsets, const int *pixels,
+                    int *bpix, double *chi);
+/* Elementwise FP64 primitives of the index path on the device (op: 0 sin,
+ * 1 cos, 2 acos, 3 atan2(x,y), 4 sqrt, 5 x/y, 6 fmodulo(x,y), 7 sincos). */
+int pisco_dev_libm(pisco_ctx *ctx, void *stream, int op, long n,
+                   const double *x, const double *y, double *out);
+
+/* ---- host-side helpers (no device needed) ------------------------------ */
+
+/* rhoMax of a beam of npix pixels: theta of pixel npix at NSIDE nside_beam
+ * (PolBeam ctor, polbeam.cpp:35-36).  Returns -1 on bad arguments. */
+double pisco_beam_rho_max(int nside_beam, long npix);
+
+/* Healpix_Base::ang2pix(pointing(theta, phi)) on the host; used to size a
+ * beam cap as scripts/fields2beams.py:102 does (npix = ang2pix(nside, R, 0)).
+ * Returns -1 on bad arguments. */
+long pisco_ang2pix(int nside, double theta, double phi);
+
+/* Healpix_Base::pix2ang on the host.  Returns PISCO_OK or an error. */
+int pisco_pix2ang(int nside, long pix, double *theta, double *phi);
+
+/* E-field magnitude and phase of a perfectly co-polarised elliptical
+ * Gaussian beam on the first npix pixels of a beam map
+ * (PolBeam::make_unpol_gaussian_elliptical_beams, polbeam.cpp:290-372;
+ * fwhm in degrees, phi0 in degrees). */
+int pisco_beam_gaussian_fields(int nside_beam, long npix, double fwhmx,
+                               double fwhmy, double phi0, double *magEco,
+                               double *phsEco, double *magEcx, double *phsEcx);
+
+/* tilde I,Q,U,V of one detector from its E fields
+ * (PolBeam::beam_from_fields, polbeam.cpp:117-184). */
+int pisco_beam_from_fields(long npix, const double *magEco, const double *phsEco,
+                           const double *magEcx, const double *phsEcx, float *I,
+                           float *Q, float *U, float *V);
+
+/* Normalised, leakage-mixed beams (PolBeam::build, polbeam.cpp:186-288):
+ * tilde = {Ia,Qa,Ua,Va,Ib,Qb,Ub,Vb}, out = {aBeams[0..3], bBeams[0..3]};
+ * dets is 'a', 'b' or 'p' (PolBeam::enabledDets). */
+int pisco_polbeam_build(long npix, int nside_beam, int nside_sky, double epsilon,
+                        char dets, const float *const tilde[8],
+                        float *const out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PISCO_H */

@@ -1,0 +1,740 @@
+/*
+ * capi.hip -- the extern "C" boundary (include/pisco.h): context, device
+ * buffers, host <-> device staging and kernel launches.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "../../include/pisco.h"
+#include "device_hpx.h"
+#include "kernels.hpp"
+
+using namespace pisco;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return fail(PISCO_ERR_DEVICE, "%s failed: %s", #expr,                 \
+                        hipGetErrorString(e_));                                   \
+    } while (0)
+
+#define CHK(expr)                                                                 \
+    do {                                                                          \
+        int rc_ = (expr);                                                         \
+        if (rc_ != PISCO_OK)                                                      \
+            return rc_;                                                           \
+    } while (0)
+
+constexpr int kMaxNside = 8192;   /* int32 pixel indices, as Healpix_Base<int> */
+
+bool valid_nside(int n) { return n >= 1 && n <= kMaxNside; }
+
+/* A grow-only device allocation. */
+template <typename T> struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;   /* elements */
+    int ensure(size_t n)
+    {
+        if (n <= cap)
+            return PISCO_OK;
+        if (p)
+            hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(PISCO_ERR_MEMORY, "hipMalloc of %zu bytes failed: %s",
+                        n * sizeof(T), hipGetErrorString(e));
+        }
+        cap = n;
+        return PISCO_OK;
+    }
+    void release()
+    {
+        if (p)
+            hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+} // namespace
+
+struct pisco_ctx {
+    pisco_run_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    bool has_sky = false, has_beam = false, has_scan = false;
+    int nside_sky = 0;
+    DevBuf<float> I, Q, U;
+    int nside_beam = 0;
+    long npix_beam = 0;
+    double rho_max = 0.0;
+    DevBuf<float4> beam;          /* [npix][2] */
+    long nsamples = 0;
+    DevBuf<float> ra, dec, pa;
+    DevBuf<float> tod;            /* 2 * nsamples */
+    DevBuf<float> cot;            /* 2 * nsamples cotangents */
+    DevBuf<double> acc;           /* FP64 accumulators */
+    DevBuf<float> fout;           /* float staging of adjoint outputs */
+    std::map<int, double2 *> ring_tabs;
+};
+
+namespace {
+
+hipStream_t pick(pisco_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+int ring_table(pisco_ctx *c, int nside, const double2 **out)
+{
+    auto it = c->ring_tabs.find(nside);
+    if (it != c->ring_tabs.end()) {
+        *out = it->second;
+        return PISCO_OK;
+    }
+    double2 *t = nullptr;
+    HIPCHK(hipMalloc((void **)&t, sizeof(double2) * 4 * (size_t)nside));
+    HIPCHK(launch_ring_table(hpx_make(nside), t, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ring_tabs[nside] = t;
+    *out = t;
+    return PISCO_OK;
+}
+
+double cos_radius(double r) { return pl_cos(r < kPi ? r : kPi); }
+
+/* The reference's memory estimate (gpuconvolver.cu:41-64), one stream. */
+size_t reference_mem_estimate(const pisco_run_config &cfg)
+{
+    size_t ptg = (size_t)cfg.ptgPerConv;
+    size_t result = ptg * sizeof(float);
+    size_t ptgbuf = 4 * ptg * sizeof(float);
+    size_t pixbuf = ptg * (size_t)cfg.pixelsPerDisc * sizeof(int);
+    size_t npixbuf = ptg * sizeof(int);
+    return result + ptgbuf + npixbuf + pixbuf;
+}
+
+int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
+               const float *U, int nside_beam, long npix_beam, const float *beam8,
+               long nsamples, const float *ra, const float *dec, const float *pa,
+               PairParams &p)
+{
+    if (!valid_nside(nside_sky) || !valid_nside(nside_beam))
+        return fail(PISCO_ERR_INVALID, "NSIDE out of range [1, %d]", kMaxNside);
+    long npb_max = 12L * nside_beam * nside_beam;
+    if (npix_beam < 1 || npix_beam >= npb_max)
+        return fail(PISCO_ERR_INVALID,
+                    "beam npix %ld must be in [1, 12*nside_beam^2 - 1]", npix_beam);
+    if (nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "negative nsamples");
+    const double2 *rt = nullptr;
+    CHK(ring_table(c, nside_sky, &rt));
+    memset(&p, 0, sizeof p);
+    p.hs = hpx_make(nside_sky);
+    p.ring_cs = rt;
+    p.I = I;
+    p.Q = Q;
+    p.U = U;
+    p.hb = hpx_make(nside_beam);
+    p.npix_beam = (int)npix_beam;
+    p.rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
+    p.cos_rmax = cos_radius(p.rho_max);
+    p.beam = (const float4 *)beam8;
+    p.ra = ra;
+    p.dec = dec;
+    p.pa = pa;
+    p.s_begin = 0;
+    p.s_end = nsamples;
+    return PISCO_OK;
+}
+
+/* Launches op over all samples in chunks of ptgPerConv (or one launch). */
+int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
+{
+    long n = p.s_end;
+    long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : n;
+    if (chunk < (1L << 20))
+        chunk = std::max<long>(chunk, std::min<long>(n, 1L << 20));
+    for (long s = 0; s < n; s += chunk) {
+        p.s_begin = s;
+        p.s_end = std::min(n, s + chunk);
+        HIPCHK(launch_pair(op, p, st));
+    }
+    return PISCO_OK;
+}
+
+int need_state(pisco_ctx *c, bool beam_needed, bool sky_needed)
+{
+    if (!c)
+        return fail(PISCO_ERR_INVALID, "NULL context");
+    if (!c->has_scan)
+        return fail(PISCO_ERR_STATE, "update_scan must be called before exec");
+    if (beam_needed && !c->has_beam)
+        return fail(PISCO_ERR_STATE, "update_beam must be called before exec");
+    if (sky_needed && !c->has_sky)
+        return fail(PISCO_ERR_STATE, "update_sky must be called before exec");
+    return PISCO_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *pisco_last_error(void) { return g_err.c_str(); }
+
+int pisco_version(char *buf, int len)
+{
+    int rt = 0;
+    hipRuntimeGetVersion(&rt);
+    if (buf && len > 0)
+        snprintf(buf, (size_t)len, "pisco4cmb_amd 0.1 gfx950 hip-runtime %d", rt);
+    return PISCO_OK;
+}
+
+double pisco_beam_rho_max(int nside_beam, long npix)
+{
+    if (!valid_nside(nside_beam) || npix < 0 || npix >= 12L * nside_beam * nside_beam)
+        return -1.0;
+    double th, ph;
+    pix2ang(hpx_make(nside_beam), (int)npix, th, ph);
+    return th;
+}
+
+long pisco_ang2pix(int nside, double theta, double phi)
+{
+    if (!valid_nside(nside) || !(theta >= 0.0 && theta <= kPi) || !(fabs(phi) < 1e300))
+        return -1;
+    return ang2pix(hpx_make(nside), theta, phi);
+}
+
+int pisco_pix2ang(int nside, long pix, double *theta, double *phi)
+{
+    if (!valid_nside(nside) || pix < 0 || pix >= 12L * nside * nside || !theta || !phi)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    pix2ang(hpx_make(nside), (int)pix, *theta, *phi);
+    return PISCO_OK;
+}
+
+int pisco_beam_gaussian_fields(int nside_beam, long npix, double fwhmx, double fwhmy,
+                               double phi0, double *magEco, double *phsEco,
+                               double *magEcx, double *phsEcx)
+{
+    if (!valid_nside(nside_beam) || npix < 0 || npix > 12L * nside_beam * nside_beam)
+        return fail(PISCO_ERR_INVALID, "bad beam geometry");
+    if (!magEco || !phsEco || !magEcx || !phsEcx)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (!(fwhmx > 0) || !(fwhmy > 0))
+        return fail(PISCO_ERR_INVALID, "FWHM must be positive");
+    Hpx h = hpx_make(nside_beam);
+    const double deg2rad = M_PI / 180.0;
+    /* FWHM = 2 sqrt(2 ln 2) sigma */
+    double sx = (deg2rad * fwhmx) / 2.35482;
+    double sy = (deg2rad * fwhmy) / 2.35482;
+    double p0 = deg2rad * phi0;
+    double ca = cos(p0), sa = sin(p0), s2 = sin(2 * p0);
+    double qa = (ca * ca) / (2 * sx * sx) + (sa * sa) / (2 * sy * sy);
+    double qb = -(s2) / (4 * sx * sx) + (s2) / (4 * sy * sy);
+    double qc = (sa * sa) / (2 * sx * sx) + (ca * ca) / (2 * sy * sy);
+    for (long i = 0; i < npix; i++) {
+        double rho, sig;
+        pix2ang(h, (int)i, rho, sig);
+        double cs = cos(sig), sn = sin(sig);
+        double val = exp(-(qa * cs * cs + 2.0 * qb * cs * sn + qc * sn * sn) * rho * rho);
+        magEco[i] = sqrt(val);
+        magEcx[i] = 0.0;
+        phsEco[i] = 0.0;
+        phsEcx[i] = 0.0;
+    }
+    return PISCO_OK;
+}
+
+int pisco_beam_from_fields(long npix, const double *magEco, const double *phsEco,
+                           const double *magEcx, const double *phsEcx, float *I, float *Q,
+                           float *U, float *V)
+{
+    if (npix < 0 || !magEco || !phsEco || !magEcx || !phsEcx || !I || !Q || !U || !V)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    for (long i = 0; i < npix; i++) {
+        /* E_co, E_cx as complex numbers (re, im) */
+        double cr = magEco[i] * cos(phsEco[i]), ci = magEco[i] * sin(phsEco[i]);
+        double xr = magEcx[i] * cos(phsEcx[i]), xi = magEcx[i] * sin(phsEcx[i]);
+        double nco = cr * cr + ci * ci, ncx = xr * xr + xi * xi;
+        double pr = cr * xr - ci * (-xi);     /* Re(E_co conj(E_cx)) */
+        double pim = cr * (-xi) + ci * xr;    /* Im(E_co conj(E_cx)) */
+        I[i] = (float)(nco + ncx);
+        Q[i] = (float)(nco - ncx);
+        U[i] = (float)(2 * pr);
+        V[i] = (float)(-2 * pim);
+    }
+    return PISCO_OK;
+}
+
+int pisco_polbeam_build(long npix, int nside_beam, int nside_sky, double epsilon, char dets,
+                        const float *const tilde[8], float *const out[8])
+{
+    if (npix < 0 || !tilde || !out || !valid_nside(nside_beam) || !valid_nside(nside_sky))
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (dets != 'a' && dets != 'b' && dets != 'p')
+        return fail(PISCO_ERR_INVALID, "enabledDets must be 'a', 'b' or 'p'");
+    for (int k = 0; k < 8; k++)
+        if (!tilde[k] || !out[k])
+            return fail(PISCO_ERR_INVALID, "NULL map %d", k);
+    const float *Ia = tilde[0], *Qa = tilde[1], *Ua = tilde[2], *Va = tilde[3];
+    const float *Ib = tilde[4], *Qb = tilde[5], *Ub = tilde[6], *Vb = tilde[7];
+    double sumIa = 0, sumIb = 0;   /* solid angles */
+    for (long i = 0; i < npix; i++) {
+        if (dets == 'a' || dets == 'p')
+            sumIa += Ia[i];
+        if (dets == 'b' || dets == 'p')
+            sumIb += Ib[i];
+    }
+    double ocmp = nside_sky / (double)nside_beam;
+    ocmp = ocmp * ocmp;
+    double aNorm = sumIa * ocmp, bNorm = sumIb * ocmp;
+    if (dets == 'a')
+        bNorm += 1e-8;
+    if (dets == 'b')
+        aNorm += 1e-8;
+    for (long i = 0; i < npix; i++) {
+        out[0][i] = (float)((Ia[i] + epsilon * Ib[i]) / aNorm);
+        out[1][i] = (float)((Qa[i] - epsilon * Qb[i]) / aNorm);
+        out[2][i] = (float)((Ua[i] - epsilon * Ub[i]) / aNorm);
+        out[3][i] = (float)((Va[i] + epsilon * Vb[i]) / aNorm);
+        out[4][i] = (float)((Ib[i] + epsilon * Ia[i]) / bNorm);
+        out[5][i] = (float)((Qb[i] - epsilon * Qa[i]) / bNorm);
+        out[6][i] = (float)((Ub[i] - epsilon * Ua[i]) / bNorm);
+        out[7][i] = (float)((Vb[i] + epsilon * Va[i]) / bNorm);
+    }
+    return PISCO_OK;
+}
+
+int pisco_create(const pisco_run_config *cfg, pisco_ctx **out)
+{
+    if (!cfg || !out)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    pisco_run_config c = *cfg;
+    c.nStreams = 1;   /* as the reference (gpuconvolver.cu:39) */
+    if (c.ptgPerConv <= 0)
+        return fail(PISCO_ERR_INVALID, "ptgPerConv must be positive");
+    if (c.pixelsPerDisc < 0)
+        return fail(PISCO_ERR_INVALID, "pixelsPerDisc must be non-negative");
+    size_t req = reference_mem_estimate(c);
+    if (!(req < c.maxMemUsage))
+        return fail(PISCO_ERR_MEMORY,
+                    "GPUConvolver has an invalid configuration: requested memory %zu "
+                    "bytes is not below maxMemUsage %zu bytes",
+                    req, c.maxMemUsage);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (c.deviceId < 0 || c.deviceId >= ndev)
+        return fail(PISCO_ERR_INVALID, "deviceId %d out of range (%d devices)",
+                    c.deviceId, ndev);
+    HIPCHK(hipSetDevice(c.deviceId));
+    pisco_ctx *ctx = new pisco_ctx();
+    ctx->cfg = c;
+    ctx->device = c.deviceId;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(PISCO_ERR_DEVICE, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return PISCO_OK;
+}
+
+void pisco_destroy(pisco_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    ctx->I.release();
+    ctx->Q.release();
+    ctx->U.release();
+    ctx->beam.release();
+    ctx->ra.release();
+    ctx->dec.release();
+    ctx->pa.release();
+    ctx->tod.release();
+    ctx->cot.release();
+    ctx->acc.release();
+    ctx->fout.release();
+    for (auto &kv : ctx->ring_tabs)
+        hipFree(kv.second);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int pisco_update_sky(pisco_ctx *c, int nside, const float *I, const float *Q,
+                     const float *U, const float *V)
+{
+    (void)V;   /* the CPU Convolver does not read V (convolver.cpp:157-164) */
+    if (!c || !I || !Q || !U)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (!valid_nside(nside))
+        return fail(PISCO_ERR_INVALID, "sky NSIDE %d out of range [1, %d]", nside,
+                    kMaxNside);
+    HIPCHK(hipSetDevice(c->device));
+    size_t n = 12UL * nside * nside;
+    CHK(c->I.ensure(n));
+    CHK(c->Q.ensure(n));
+    CHK(c->U.ensure(n));
+    HIPCHK(hipMemcpyAsync(c->I.p, I, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->Q.p, Q, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->U.p, U, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const double2 *rt;
+    CHK(ring_table(c, nside, &rt));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nside_sky = nside;
+    c->has_sky = true;
+    return PISCO_OK;
+}
+
+int pisco_update_beam(pisco_ctx *c, int nside_beam, long npix, const float *const a[4],
+                      const float *const b[4])
+{
+    if (!c || !a || !b)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    for (int k = 0; k < 3; k++)
+        if (!a[k] || !b[k])
+            return fail(PISCO_ERR_INVALID, "beam maps 0..2 of a and b are required");
+    if (!valid_nside(nside_beam))
+        return fail(PISCO_ERR_INVALID, "beam NSIDE %d out of range", nside_beam);
+    if (npix < 1 || npix >= 12L * nside_beam * nside_beam)
+        return fail(PISCO_ERR_INVALID, "beam npix %ld out of range", npix);
+    HIPCHK(hipSetDevice(c->device));
+    CHK(c->beam.ensure(2 * (size_t)npix));
+    /* host transpose into the [npix][8] layout (as update_beam does into
+     * float4, gpuconvolver.cu:302-340) */
+    float *h = nullptr;
+    HIPCHK(hipHostMalloc((void **)&h, sizeof(float) * 8 * (size_t)npix, 0));
+    for (long i = 0; i < npix; i++) {
+        float *o = h + 8 * i;
+        o[0] = a[0][i]; o[1] = a[1][i]; o[2] = a[2][i]; o[3] = a[3] ? a[3][i] : 0.f;
+        o[4] = b[0][i]; o[5] = b[1][i]; o[6] = b[2][i]; o[7] = b[3] ? b[3][i] : 0.f;
+    }
+    hipError_t e = hipMemcpyAsync(c->beam.p, h, sizeof(float) * 8 * (size_t)npix,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    hipHostFree(h);
+    HIPCHK(e);
+    c->nside_beam = nside_beam;
+    c->npix_beam = npix;
+    c->rho_max = pisco_beam_rho_max(nside_beam, npix);
+    c->has_beam = true;
+    return PISCO_OK;
+}
+
+int pisco_update_scan(pisco_ctx *c, long nsamples, const float *ra, const float *dec,
+                      const float *pa)
+{
+    if (!c || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "negative nsamples");
+    HIPCHK(hipSetDevice(c->device));
+    size_t n = (size_t)nsamples;
+    CHK(c->ra.ensure(n));
+    CHK(c->dec.ensure(n));
+    CHK(c->pa.ensure(n));
+    if (n) {
+        HIPCHK(hipMemcpyAsync(c->ra.p, ra, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->dec.p, dec, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->pa.p, pa, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->nsamples = nsamples;
+    c->has_scan = true;
+    return PISCO_OK;
+}
+
+int pisco_exec_convolution(pisco_ctx *c, char polflag, float *tod_a, float *tod_b)
+{
+    if (!c)
+        return fail(PISCO_ERR_INVALID, "NULL context");
+    if (polflag != 'a' && polflag != 'b' && polflag != 'p')
+        return fail(PISCO_ERR_INVALID, "valid polFlags are 'a', 'b' and 'p'");
+    if ((polflag != 'b' && !tod_a) || (polflag != 'a' && !tod_b))
+        return fail(PISCO_ERR_INVALID, "NULL output for the requested polFlag");
+    CHK(need_state(c, true, true));
+    HIPCHK(hipSetDevice(c->device));
+    long n = c->nsamples;
+    if (n == 0)
+        return PISCO_OK;
+    CHK(c->tod.ensure(2 * (size_t)n));
+    PairParams p;
+    CHK(setup_pair(c, c->nside_sky, c->I.p, c->Q.p, c->U.p, c->nside_beam, c->npix_beam,
+                   (const float *)c->beam.p, n, c->ra.p, c->dec.p, c->pa.p, p));
+    p.tod_a = c->tod.p;
+    p.tod_b = c->tod.p + n;
+    CHK(run_pair(c, OP_FWD, p, c->stream));
+    if (polflag != 'b')
+        HIPCHK(hipMemcpyAsync(tod_a, c->tod.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (polflag != 'a')
+        HIPCHK(hipMemcpyAsync(tod_b, c->tod.p + n, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PISCO_OK;
+}
+
+int pisco_exec_adjoint_map(pisco_ctx *c, const float *g_a, const float *g_b, float *gI,
+                           float *gQ, float *gU)
+{
+    if (!c || !gI || !gQ || !gU)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    CHK(need_state(c, true, true));
+    HIPCHK(hipSetDevice(c->device));
+    long n = c->nsamples;
+    size_t npix = 12UL * c->nside_sky * c->nside_sky;
+    CHK(c->cot.ensure(2 * (size_t)n + 1));
+    if (n) {
+        if (g_a)
+            HIPCHK(hipMemcpyAsync(c->cot.p, g_a, n * 4, hipMemcpyHostToDevice, c->stream));
+        if (g_b)
+            HIPCHK(hipMemcpyAsync(c->cot.p + n, g_b, n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    CHK(c->fout.ensure(3 * npix));
+    CHK(pisco_dev_adjoint_map(c, c->stream, c->nside_sky, c->nside_beam, c->npix_beam,
+                              (const float *)c->beam.p, n, c->ra.p, c->dec.p, c->pa.p,
+                              g_a ? c->cot.p : nullptr, g_b ? c->cot.p + n : nullptr,
+                              c->fout.p, c->fout.p + npix, c->fout.p + 2 * npix));
+    HIPCHK(hipMemcpyAsync(gI, c->fout.p, npix * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gQ, c->fout.p + npix, npix * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gU, c->fout.p + 2 * npix, npix * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PISCO_OK;
+}
+
+int pisco_exec_adjoint_beam(pisco_ctx *c, const float *g_a, const float *g_b,
+                            float *const ga[4], float *const gb[4])
+{
+    if (!c || !ga || !gb)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    CHK(need_state(c, true, true));
+    HIPCHK(hipSetDevice(c->device));
+    long n = c->nsamples;
+    long nb = c->npix_beam;
+    CHK(c->cot.ensure(2 * (size_t)n + 1));
+    if (n) {
+        if (g_a)
+            HIPCHK(hipMemcpyAsync(c->cot.p, g_a, n * 4, hipMemcpyHostToDevice, c->stream));
+        if (g_b)
+            HIPCHK(hipMemcpyAsync(c->cot.p + n, g_b, n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    CHK(c->fout.ensure(8 * (size_t)nb));
+    CHK(pisco_dev_adjoint_beam(c, c->stream, c->nside_sky, c->I.p, c->Q.p, c->U.p,
+                               c->nside_beam, nb, n, c->ra.p, c->dec.p, c->pa.p,
+                               g_a ? c->cot.p : nullptr, g_b ? c->cot.p + n : nullptr,
+                               c->fout.p));
+    /* [nb][8] -> 8 maps on the host */
+    float *h = nullptr;
+    HIPCHK(hipHostMalloc((void **)&h, sizeof(float) * 8 * (size_t)nb, 0));
+    hipError_t e = hipMemcpyAsync(h, c->fout.p, sizeof(float) * 8 * (size_t)nb,
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) {
+        for (long i = 0; i < nb; i++)
+            for (int k = 0; k < 4; k++) {
+                if (ga[k]) ga[k][i] = h[8 * i + k];
+                if (gb[k]) gb[k][i] = h[8 * i + 4 + k];
+            }
+    }
+    hipHostFree(h);
+    HIPCHK(e);
+    return PISCO_OK;
+}
+
+/* ---- device-resident entry points -------------------------------------- */
+
+int pisco_dev_pack_beam(pisco_ctx *c, void *stream, long npix, const float *const maps[8],
+                        float *beam8)
+{
+    if (!c || !maps || !beam8)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    for (int k = 0; k < 8; k++)
+        if (k != 3 && k != 7 && !maps[k])
+            return fail(PISCO_ERR_INVALID, "beam map %d is NULL", k);
+    if (npix < 1)
+        return fail(PISCO_ERR_INVALID, "npix must be positive");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_pack_beam(npix, maps, (float4 *)beam8, pick(c, stream)));
+    return PISCO_OK;
+}
+
+int pisco_dev_forward(pisco_ctx *c, void *stream, int nside_sky, const float *I,
+                      const float *Q, const float *U, int nside_beam, long npix_beam,
+                      const float *beam8, long nsamples, const float *ra, const float *dec,
+                      const float *pa, float *tod_a, float *tod_b)
+{
+    if (!c || !I || !Q || !U || !beam8 || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    PairParams p;
+    CHK(setup_pair(c, nside_sky, I, Q, U, nside_beam, npix_beam, beam8, nsamples, ra, dec,
+                   pa, p));
+    p.tod_a = tod_a;
+    p.tod_b = tod_b;
+    return run_pair(c, OP_FWD, p, pick(c, stream));
+}
+
+int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_beam,
+                          long npix_beam, const float *beam8, long nsamples,
+                          const float *ra, const float *dec, const float *pa,
+                          const float *g_a, const float *g_b, float *gI, float *gQ,
+                          float *gU)
+{
+    if (!c || !beam8 || !gI || !gQ || !gU || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = pick(c, stream);
+    PairParams p;
+    CHK(setup_pair(c, nside_sky, nullptr, nullptr, nullptr, nside_beam, npix_beam, beam8,
+                   nsamples, ra, dec, pa, p));
+    long npix = 12L * nside_sky * nside_sky;
+    CHK(c->acc.ensure(3 * (size_t)npix));
+    HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
+    p.g_a = g_a;
+    p.g_b = g_b;
+    p.gI = c->acc.p;
+    p.gQ = c->acc.p + npix;
+    p.gU = c->acc.p + 2 * npix;
+    if (g_a || g_b)
+        CHK(run_pair(c, OP_ADJ_MAP, p, st));
+    HIPCHK(launch_f64_to_f32(p.gI, gI, npix, st));
+    HIPCHK(launch_f64_to_f32(p.gQ, gQ, npix, st));
+    HIPCHK(launch_f64_to_f32(p.gU, gU, npix, st));
+    return PISCO_OK;
+}
+
+int pisco_dev_adjoint_beam(pisco_ctx *c, void *stream, int nside_sky, const float *I,
+                           const float *Q, const float *U, int nside_beam, long npix_beam,
+                           long nsamples, const float *ra, const float *dec,
+                           const float *pa, const float *g_a, const float *g_b,
+                           float *gbeam8)
+{
+    if (!c || !I || !Q || !U || !gbeam8 || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = pick(c, stream);
+    PairParams p;
+    /* beam values are not read by the beam adjoint; pass a valid pointer */
+    CHK(setup_pair(c, nside_sky, I, Q, U, nside_beam, npix_beam, gbeam8, nsamples, ra,
+                   dec, pa, p));
+    CHK(c->acc.ensure(8 * (size_t)npix_beam));
+    HIPCHK(launch_zero_f64(c->acc.p, 8 * npix_beam, st));
+    p.g_a = g_a;
+    p.g_b = g_b;
+    p.gbeam = c->acc.p;
+    if (g_a || g_b)
+        CHK(run_pair(c, OP_ADJ_BEAM, p, st));
+    HIPCHK(launch_f64_to_f32(c->acc.p, gbeam8, 8 * npix_beam, st));
+    return PISCO_OK;
+}
+
+int pisco_dev_pix2ang(pisco_ctx *c, void *stream, int nside, long n, const int *pix,
+                      double *theta, double *phi)
+{
+    if (!c || !valid_nside(nside) || n < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (n == 0)
+        return PISCO_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_pix2ang(hpx_make(nside), n, pix, theta, phi, pick(c, stream)));
+    return PISCO_OK;
+}
+
+int pisco_dev_ang2pix(pisco_ctx *c, void *stream, int nside, long n, const double *theta,
+                      const double *phi, int *pix)
+{
+    if (!c || !valid_nside(nside) || n < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (n == 0)
+        return PISCO_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_ang2pix(hpx_make(nside), n, theta, phi, pix, pick(c, stream)));
+    return PISCO_OK;
+}
+
+int pisco_dev_get_interpol(pisco_ctx *c, void *stream, int nside, long n,
+                           const double *theta, const double *phi, int *pix, double *wgt)
+{
+    if (!c || !valid_nside(nside) || n < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (n == 0)
+        return PISCO_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_interpol(hpx_make(nside), n, theta, phi, pix, wgt, pick(c, stream)));
+    return PISCO_OK;
+}
+
+int pisco_dev_query_disc(pisco_ctx *c, void *stream, int nside, long n, const float *ra,
+                         const float *dec, double radius, const long *offsets, int *counts,
+                         int *pixels)
+{
+    if (!c || !valid_nside(nside) || n < 0 || !(radius >= 0))
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (pixels && !offsets)
+        return fail(PISCO_ERR_INVALID, "pixels requires offsets");
+    if (n == 0)
+        return PISCO_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_disc(hpx_make(nside), n, ra, dec, radius, cos_radius(radius), offsets,
+                       counts, pixels, pick(c, stream)));
+    return PISCO_OK;
+}
+
+int pisco_dev_match(pisco_ctx *c, void *stream, int nside_sky, int nside_beam, long n,
+                    const float *ra, const float *dec, const float *pa, const long *offsets,
+                    const int *pixels, int *bpix, double *chi)
+{
+    if (!c || !valid_nside(nside_sky) || !valid_nside(nside_beam) || n < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (n == 0)
+        return PISCO_OK;
+    HIPCHK(hipSetDevice(c->device));
+    const double2 *rt;
+    CHK(ring_table(c, nside_sky, &rt));
+    HIPCHK(launch_match(hpx_make(nside_sky), rt, hpx_make(nside_beam), n, ra, dec, pa,
+                        offsets, pixels, bpix, chi, pick(c, stream)));
+    return PISCO_OK;
+}
+
+int pisco_dev_libm(pisco_ctx *c, void *stream, int op, long n, const double *x,
+                   const double *y, double *out)
+{
+    if (!c || op < 0 || op > 7 || n < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (n == 0)
+        return PISCO_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_libm(op, n, x, y, out, pick(c, stream)));
+    return PISCO_OK;
+}
+
+} /* extern "C" */

@@ -1,0 +1,481 @@
+/*
+ * device_hpx.h -- HEALPix RING geometry and the per-visit beam-frame mapping
+ * for gfx950 kernels.
+ *
+ * Semantics follow the CPU path of PISCO (Convolver, convolver.cpp:74-183),
+ * which takes pix2ang / ang2pix / query_disc from healpix_cxx's Healpix_Base
+ * (RING scheme, non-inclusive disc), and the rotation of
+ * SphericalTransformations::rho_sigma_chi_pix (sphtrigo.cpp:4-55).  The device
+ * port of those routines in the reference (cudaHealpix.cu:30-259) differs from
+ * the CPU library in three places (its fmod, its polar "sth" rule and the
+ * missing imodulo); the CPU behaviour is the one reproduced here.
+ *
+ * All FP64 arithmetic is written so that, compiled with -ffp-contract=off and
+ * evaluated with pisco_libm.h, it yields the same bits as the CPU oracle:
+ * pixel indices are therefore bit-exact, not merely close.
+ */
+#ifndef PISCO_DEVICE_HPX_H
+#define PISCO_DEVICE_HPX_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pisco_libm.h"
+
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
+namespace pisco {
+
+constexpr double kPi = 3.141592653589793238462643383279502884197;
+constexpr double kTwoPi = 6.283185307179586476925286766559005768394;
+constexpr double kHalfPi = 1.570796326794896619231321691639751442099;
+constexpr double kInvHalfPi = 0.6366197723675813430755350534900574;
+constexpr double kInvTwoPi = 1.0 / 6.283185307179586476925286766559005768394;
+constexpr double kTwoThird = 2.0 / 3.0;
+
+/* Geometry of one RING-ordered HEALPix map (NSIDE <= 8192, int32 pixels,
+ * as Healpix_Base<int>). */
+struct Hpx {
+    int nside;
+    int ncap;   /* pixels in the north polar cap */
+    int npix;
+    double fact1, fact2;
+};
+
+__host__ __device__ inline Hpx hpx_make(int nside)
+{
+    Hpx h;
+    h.nside = nside;
+    h.ncap = 2 * (nside * nside - nside);
+    h.npix = 12 * nside * nside;
+    h.fact2 = 4. / h.npix;
+    h.fact1 = (nside << 1) * h.fact2;
+    return h;
+}
+
+/* fmod(x, y) for finite x and y > 0, exact (IEEE fmod is exact, and so is
+ * this: the candidate quotient k is off by at most one and x - k*y is
+ * representable for the right k, so one fma returns it unrounded). */
+__host__ __device__ inline double exact_fmod(double x, double y)
+{
+    double ax = fabs(x);
+    if (ax < y)
+        return x;
+    double k = trunc(ax / y);
+    double r = fma(-k, y, ax);
+    if (r < 0.0) {
+        k -= 1.0;
+        r = fma(-k, y, ax);
+    } else if (r >= y) {
+        k += 1.0;
+        r = fma(-k, y, ax);
+    }
+    return copysign(r, x);
+}
+
+/* healpix_cxx fmodulo: result in [0, v2). */
+__host__ __device__ inline double fmodulo(double v1, double v2)
+{
+    if (v1 >= 0)
+        return (v1 < v2) ? v1 : exact_fmod(v1, v2);
+    double tmp = exact_fmod(v1, v2) + v2;
+    return (tmp == v2) ? 0. : tmp;
+}
+
+__host__ __device__ inline int isqrt_i(int v)
+{
+    return (int)sqrt((double)v + 0.5);
+}
+
+/* Ring index (1..4N-1) of a RING pixel. */
+__host__ __device__ inline int pix2ring(const Hpx &h, int pix)
+{
+    if (pix < h.ncap)
+        return (1 + isqrt_i(1 + 2 * pix)) >> 1;
+    if (pix < h.npix - h.ncap)
+        return (pix - h.ncap) / (4 * h.nside) + h.nside;
+    int ip = h.npix - pix;
+    return 4 * h.nside - ((1 + isqrt_i(2 * ip - 1)) >> 1);
+}
+
+/* First pixel, number of pixels and shift flag of ring r
+ * (Healpix_Base::get_ring_info_small). */
+__host__ __device__ inline void ring_info(const Hpx &h, int r, int &startpix, int &ringpix,
+                                 bool &shifted)
+{
+    if (r < h.nside) {
+        shifted = true;
+        ringpix = 4 * r;
+        startpix = 2 * r * (r - 1);
+    } else if (r < 3 * h.nside) {
+        shifted = ((r - h.nside) & 1) == 0;
+        ringpix = 4 * h.nside;
+        startpix = h.ncap + (r - h.nside) * ringpix;
+    } else {
+        shifted = true;
+        int nr = 4 * h.nside - r;
+        ringpix = 4 * nr;
+        startpix = h.npix - 2 * nr * (nr + 1);
+    }
+}
+
+/* z and (if |z| > 0.99) sin(theta) of the pixels of ring r, as pix2loc
+ * computes them. */
+__host__ __device__ inline void ring_z(const Hpx &h, int r, double &z, double &sth,
+                              bool &have_sth)
+{
+    have_sth = false;
+    sth = 0.0;
+    int n = h.nside;
+    if (r < n) {
+        double tmp = (r * r) * h.fact2;
+        z = 1.0 - tmp;
+        if (z > 0.99) {
+            sth = sqrt(tmp * (2.0 - tmp));
+            have_sth = true;
+        }
+    } else if (r <= 3 * n) {
+        z = (2 * n - r) * h.fact1;
+    } else {
+        int ir = 4 * n - r;
+        double tmp = (ir * ir) * h.fact2;
+        z = tmp - 1.0;
+        if (z < -0.99) {
+            sth = sqrt(tmp * (2.0 - tmp));
+            have_sth = true;
+        }
+    }
+}
+
+/* theta of ring r (pix2ang). */
+__host__ __device__ inline double ring_theta(const Hpx &h, int r)
+{
+    double z, sth;
+    bool hs;
+    ring_z(h, r, z, sth, hs);
+    return hs ? pl_atan2(sth, z) : pl_acos(z);
+}
+
+/* phi of pixel pix lying in ring r with first pixel startpix (pix2loc). */
+__host__ __device__ inline double pixel_phi(const Hpx &h, int r, int pix, int startpix)
+{
+    int iphi = pix - startpix + 1;
+    int n = h.nside;
+    if (r < n)
+        return (iphi - 0.5) * kHalfPi / r;
+    if (r <= 3 * n) {
+        double fodd = ((r + n) & 1) ? 1 : 0.5;
+        return (iphi - fodd) * kPi * 0.75 * h.fact1;
+    }
+    return (iphi - 0.5) * kHalfPi / (4 * n - r);
+}
+
+__host__ __device__ inline void pix2ang(const Hpx &h, int pix, double &theta, double &phi)
+{
+    int r = pix2ring(h, pix);
+    int sp, nr;
+    bool sh;
+    ring_info(h, r, sp, nr, sh);
+    theta = ring_theta(h, r);
+    phi = pixel_phi(h, r, pix, sp);
+}
+
+/* Healpix_Base::loc2pix, RING. */
+__host__ __device__ inline int loc2pix(const Hpx &h, double z, double phi, double sth,
+                              bool have_sth)
+{
+    double za = fabs(z);
+    double tt = fmodulo(phi * kInvHalfPi, 4.0);
+    int nside = h.nside;
+    if (za <= kTwoThird) {
+        int nl4 = 4 * nside;
+        double temp1 = nside * (0.5 + tt);
+        double temp2 = nside * z * 0.75;
+        int jp = (int)(temp1 - temp2);
+        int jm = (int)(temp1 + temp2);
+        int ir = nside + 1 + jp - jm;
+        int kshift = 1 - (ir & 1);
+        int t1 = jp + jm - nside + kshift + 1 + nl4 + nl4;
+        int ip = (t1 >> 1) % nl4;
+        return h.ncap + (ir - 1) * nl4 + ip;
+    }
+    double tp = tt - (int)tt;
+    double tmp = ((za < 0.99) || (!have_sth)) ? nside * sqrt(3 * (1 - za))
+                                              : nside * sth / sqrt((1. + za) / 3.);
+    int jp = (int)(tp * tmp);
+    int jm = (int)((1.0 - tp) * tmp);
+    int ir = jp + jm + 1;
+    int ip = (int)(tt * ir);
+    return (z > 0) ? 2 * ir * (ir - 1) + ip : h.npix - 2 * ir * (ir + 1) + ip;
+}
+
+/* Healpix_Base::ang2pix(pointing): sin(theta) is used only within 0.01 rad
+ * of a pole. */
+__host__ __device__ inline int ang2pix(const Hpx &h, double theta, double phi)
+{
+    if ((theta < 0.01) || (theta > 3.14159 - 0.01)) {
+        double s, c;
+        pl_sincos(theta, &s, &c);
+        return loc2pix(h, c, phi, s, true);
+    }
+    return loc2pix(h, pl_cos(theta), phi, 0., false);
+}
+
+__host__ __device__ inline int ring_above(const Hpx &h, double z)
+{
+    double az = fabs(z);
+    if (az <= kTwoThird)
+        return (int)(h.nside * (2 - 1.5 * z));
+    int iring = (int)(h.nside * sqrt(3 * (1 - az)));
+    return (z > 0) ? iring : 4 * h.nside - iring - 1;
+}
+
+__host__ __device__ inline double ring2z(const Hpx &h, int ring)
+{
+    if (ring < h.nside)
+        return 1 - ring * ring * h.fact2;
+    if (ring <= 3 * h.nside)
+        return (2 * h.nside - ring) * h.fact1;
+    ring = 4 * h.nside - ring;
+    return ring * ring * h.fact2 - 1;
+}
+
+/* ---- query_disc, decomposed by ring ------------------------------------ */
+
+/* Per-pointing part of the non-inclusive RING query_disc
+ * (Healpix_Base::query_disc with fact = 0). Rings are r_first..r_last;
+ * rings outside [irmin, irmax] are whole (the polar blocks). */
+struct Disc {
+    double phi;       /* normalised phi of the centre */
+    double z0, xa;
+    double cosr;
+    int irmin, irmax;
+    int r_first, r_last;
+};
+
+__host__ __device__ inline void disc_setup(const Hpx &h, double theta, double phi,
+                                  double radius, double cosr, Disc &d)
+{
+    /* pointing::normalize */
+    theta = fmodulo(theta, kTwoPi);
+    if (theta > kPi) {
+        phi += kPi;
+        theta = kTwoPi - theta;
+    }
+    phi = fmodulo(phi, kTwoPi);
+    d.phi = phi;
+    d.cosr = cosr;
+    if (radius >= kPi) {            /* whole sky */
+        d.irmin = 4 * h.nside;
+        d.irmax = 0;
+        d.r_first = 1;
+        d.r_last = 4 * h.nside - 1;
+        d.z0 = 0;
+        d.xa = 0;
+        return;
+    }
+    d.z0 = pl_cos(theta);
+    d.xa = 1. / sqrt((1 - d.z0) * (1 + d.z0));
+    double rlat1 = theta - radius;
+    d.irmin = ring_above(h, pl_cos(rlat1)) + 1;
+    bool north = (rlat1 <= 0) && (d.irmin > 1);
+    double rlat2 = theta + radius;
+    d.irmax = ring_above(h, pl_cos(rlat2));
+    bool south = (rlat2 >= kPi) && (d.irmax + 1 < 4 * h.nside);
+    d.r_first = north ? 1 : d.irmin;
+    d.r_last = south ? 4 * h.nside - 1 : d.irmax;
+}
+
+/* Pixel ranges of ring r inside the disc, in ascending order: returns the
+ * number of ranges (0..2); range k is [pix0[k], pix0[k] + len[k]). */
+__host__ __device__ inline int disc_ring(const Hpx &h, const Disc &d, int r, int pix0[2],
+                                int len[2])
+{
+    int ipix1, nr;
+    bool shifted;
+    ring_info(h, r, ipix1, nr, shifted);
+    if (r < d.irmin || r > d.irmax) {
+        pix0[0] = ipix1;
+        len[0] = nr;
+        return 1;
+    }
+    double z = ring2z(h, r);
+    double x = (d.cosr - z * d.z0) * d.xa;
+    double ysq = 1 - z * z - x * x;
+    double dphi = (ysq <= 0) ? 0.0 : pl_atan2(sqrt(ysq), x);
+    if (!(dphi > 0))
+        return 0;
+    double shift = shifted ? 0.5 : 0.;
+    double a = nr * kInvTwoPi * (d.phi - dphi) - shift;
+    double b = nr * kInvTwoPi * (d.phi + dphi) - shift;
+    int ip_lo = ((a >= 0) ? (int)a : (int)a - 1) + 1;
+    int ip_hi = (b >= 0) ? (int)b : (int)b - 1;
+    if (ip_lo > ip_hi)
+        return 0;
+    if (ip_hi >= nr) {
+        ip_lo -= nr;
+        ip_hi -= nr;
+    }
+    if (ip_lo < 0) {
+        int n = 0;
+        if (ip_hi + 1 > 0) {
+            pix0[n] = ipix1;
+            len[n] = ip_hi + 1;
+            n++;
+        }
+        pix0[n] = ipix1 + ip_lo + nr;
+        len[n] = -ip_lo;
+        return n + 1;
+    }
+    pix0[0] = ipix1 + ip_lo;
+    len[0] = ip_hi - ip_lo + 1;
+    return 1;
+}
+
+/* ---- the beam-frame mapping of one visit -------------------------------- */
+
+/* Per-sample constants of the rotation (sphtrigo.cpp:29-30 evaluated once
+ * per sample: they depend only on the beam centre). */
+struct Centre {
+    double ra, dec, psi;   /* ra, dec of the beam centre; psi = -pa */
+    double cdc, sdc;
+};
+
+__host__ __device__ inline Centre centre_make(float ra, float dec, float pa)
+{
+    Centre c;
+    c.ra = (double)ra;
+    c.dec = (double)dec;
+    c.psi = -(double)pa;
+    pl_sincos(c.dec, &c.sdc, &c.cdc);
+    return c;
+}
+
+/* Beam pixel (ang2pix of (rho, sigma) on the beam map) and chi for the sky
+ * pixel at (ra_pix, cos/sin dec_pix): rho_sigma_chi_pix (sphtrigo.cpp:27-54)
+ * followed by Healpix_Base::ang2pix (convolver.cpp:148-149). */
+__host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double ra_pix,
+                                 double cdp, double sdp, double &chi)
+{
+    double sd, cd;
+    pl_sincos(ra_pix - c.ra, &sd, &cd);
+    double crho = c.sdc * sdp + c.cdc * cdp * cd;
+    double rho = (crho >= 1.0) ? 0.0 : pl_acos(crho);
+    double alpha, beta;
+    if (rho > 1.0e-6) {
+        beta = pl_atan2(sd * c.cdc, c.sdc * cdp - c.cdc * sdp * cd);
+        alpha = pl_atan2(sd * cdp, sdp * c.cdc - cdp * c.sdc * cd);
+    } else {
+        beta = kHalfPi;
+        alpha = kHalfPi;
+    }
+    double sigma = kPi - alpha - c.psi;
+    if (sigma < 0.0)
+        sigma += 2.0 * kPi;
+    else if (sigma > 2.0 * kPi)
+        sigma -= 2.0 * kPi;
+    chi = beta - sigma;
+    if (chi < -kPi)
+        chi += 2.0 * kPi;
+    return ang2pix(hb, rho, sigma);
+}
+
+/* get_interpol (Healpix_Base::get_interpol; the device port in the reference
+ * is cudaHealpix.cu:198-259). */
+__host__ __device__ inline void ring_info2(const Hpx &h, int ring, int &startpix,
+                                  int &ringpix, double &theta, bool &shifted)
+{
+    int northring = (ring > 2 * h.nside) ? 4 * h.nside - ring : ring;
+    if (northring < h.nside) {
+        double tmp = northring * northring * h.fact2;
+        double costheta = 1 - tmp;
+        double sintheta = sqrt(tmp * (2 - tmp));
+        theta = pl_atan2(sintheta, costheta);
+        ringpix = 4 * northring;
+        shifted = true;
+        startpix = 2 * northring * (northring - 1);
+    } else {
+        theta = pl_acos((2 * h.nside - northring) * h.fact1);
+        ringpix = 4 * h.nside;
+        shifted = ((northring - h.nside) & 1) == 0;
+        startpix = h.ncap + (northring - h.nside) * ringpix;
+    }
+    if (northring != ring) {
+        theta = kPi - theta;
+        startpix = h.npix - startpix - ringpix;
+    }
+}
+
+__host__ __device__ inline void get_interpol(const Hpx &h, double theta, double phi,
+                                    int pix[4], double wgt[4])
+{
+    double z = pl_cos(theta);
+    int ir1 = ring_above(h, z);
+    int ir2 = ir1 + 1;
+    double theta1 = 0, theta2 = 0, w1, tmp, dphi;
+    int sp, nr, i1, i2;
+    bool shift;
+    if (ir1 > 0) {
+        ring_info2(h, ir1, sp, nr, theta1, shift);
+        dphi = kTwoPi / nr;
+        tmp = (phi / dphi - .5 * shift);
+        i1 = (tmp < 0) ? (int)tmp - 1 : (int)tmp;
+        w1 = (phi - (i1 + .5 * shift) * dphi) / dphi;
+        i2 = i1 + 1;
+        if (i1 < 0) i1 += nr;
+        if (i2 >= nr) i2 -= nr;
+        pix[0] = sp + i1;
+        pix[1] = sp + i2;
+        wgt[0] = 1 - w1;
+        wgt[1] = w1;
+    }
+    if (ir2 < (4 * h.nside)) {
+        ring_info2(h, ir2, sp, nr, theta2, shift);
+        dphi = kTwoPi / nr;
+        tmp = (phi / dphi - .5 * shift);
+        i1 = (tmp < 0) ? (int)tmp - 1 : (int)tmp;
+        w1 = (phi - (i1 + .5 * shift) * dphi) / dphi;
+        i2 = i1 + 1;
+        if (i1 < 0) i1 += nr;
+        if (i2 >= nr) i2 -= nr;
+        pix[2] = sp + i1;
+        pix[3] = sp + i2;
+        wgt[2] = 1 - w1;
+        wgt[3] = w1;
+    }
+    if (ir1 == 0) {
+        double wtheta = theta / theta2;
+        wgt[2] *= wtheta;
+        wgt[3] *= wtheta;
+        double fac = (1 - wtheta) * 0.25;
+        wgt[0] = fac;
+        wgt[1] = fac;
+        wgt[2] += fac;
+        wgt[3] += fac;
+        pix[0] = (pix[2] + 2) & 3;
+        pix[1] = (pix[3] + 2) & 3;
+    } else if (ir2 == 4 * h.nside) {
+        double wtheta = (theta - theta1) / (kPi - theta1);
+        wgt[0] *= (1 - wtheta);
+        wgt[1] *= (1 - wtheta);
+        double fac = wtheta * 0.25;
+        wgt[0] += fac;
+        wgt[1] += fac;
+        wgt[2] = fac;
+        wgt[3] = fac;
+        pix[2] = ((pix[0] + 2) & 3) + h.npix - 4;
+        pix[3] = ((pix[1] + 2) & 3) + h.npix - 4;
+    } else {
+        double wtheta = (theta - theta1) / (theta2 - theta1);
+        wgt[0] *= (1 - wtheta);
+        wgt[1] *= (1 - wtheta);
+        wgt[2] *= wtheta;
+        wgt[3] *= wtheta;
+    }
+}
+
+} /* namespace pisco */
+
+#endif /* PISCO_DEVICE_HPX_H */

@@ -1,0 +1,465 @@
+/*
+ * kernels.hip -- gfx950 kernels of the pixel-space beam convolution.
+ *
+ * Work mapping (CDNA4: 64-lane wavefronts, 4 waves per 256-thread block):
+ *   one wavefront per pointing sample.  The wave first runs the ring part of
+ *   query_disc with one ring per lane (64 rings per pass), turns the 0-2 pixel
+ *   runs of each ring into a compact run table in LDS with a wave prefix sum,
+ *   and then walks the flattened visit list lane-strided, so that consecutive
+ *   lanes read consecutive sky pixels of a ring (coalesced) while the beam
+ *   reads are gathers.  Each visit recomputes the FP64 beam-frame mapping
+ *   (rho, sigma, chi -> beam pixel) exactly as the CPU path does; sums are
+ *   reduced across the wave with shuffles and written by a single lane, so
+ *   the forward is deterministic and needs no atomics.
+ *
+ * Blocks are remapped so that each of the 8 XCDs (each with its own L2) works
+ * on one contiguous stretch of the scan: consecutive samples share most of
+ * their disc pixels, and this keeps those reuses inside one L2.
+ */
+#include <hip/hip_runtime.h>
+
+#include "device_hpx.h"
+#include "kernels.hpp"
+
+namespace pisco {
+
+constexpr int kWaveSize = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWaveSize * kWavesPerBlock;
+constexpr int kMaxRuns = 2 * kWaveSize;   /* per ring pass: <= 2 runs per ring */
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline int wave_incl_scan(int x, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < kWaveSize; d <<= 1) {
+        int y = __shfl_up(x, d, kWaveSize);
+        if (lane >= d)
+            x += y;
+    }
+    return x;
+}
+
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int m = kWaveSize / 2; m > 0; m >>= 1)
+        x += __shfl_xor(x, m, kWaveSize);
+    return x;
+}
+
+/* Logical block index such that XCD x (hardware dispatches blocks round-robin
+ * over the 8 XCDs) receives a contiguous range of logical blocks. */
+__device__ inline long xcd_remap(long b, long nblk)
+{
+    const long nx = 8;
+    long q = nblk / nx, r = nblk % nx;
+    long x = b % nx, k = b / nx;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+/* Run table of one wave for one pass over (up to) 64 rings. */
+struct RunTable {
+    int pref[kMaxRuns + 1];  /* visit offset of each run; pref[nrun] = total */
+    int pix0[kMaxRuns];
+    int ring[kMaxRuns];
+};
+
+/* Builds the run table for rings r0 .. r0+63 (clipped to the disc); returns
+ * the number of visits of the pass. */
+__device__ inline int build_runs(const Hpx &hs, const Disc &d, int r0, int lane,
+                                 RunTable &t)
+{
+    int r = r0 + lane;
+    int p0[2] = {0, 0}, ln[2] = {0, 0};
+    int ns = 0;
+    if (r <= d.r_last)
+        ns = disc_ring(hs, d, r, p0, ln);
+    int cnt = ns;
+    int tot = (ns > 0 ? ln[0] : 0) + (ns > 1 ? ln[1] : 0);
+    int cnt_in = wave_incl_scan(cnt, lane);
+    int tot_in = wave_incl_scan(tot, lane);
+    int cbase = cnt_in - cnt;
+    int off = tot_in - tot;
+    for (int k = 0; k < ns; k++) {
+        t.pix0[cbase + k] = p0[k];
+        t.ring[cbase + k] = r;
+        t.pref[cbase + k] = off;
+        off += ln[k];
+    }
+    int nrun = __shfl(cnt_in, kWaveSize - 1, kWaveSize);
+    int total = __shfl(tot_in, kWaveSize - 1, kWaveSize);
+    if (lane == 0)
+        t.pref[nrun] = total;
+    wave_sync();
+    return total;
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
+{
+    __shared__ RunTable tables[kWavesPerBlock];
+    const int wave = threadIdx.x / kWaveSize;
+    const int lane = threadIdx.x % kWaveSize;
+    RunTable &t = tables[wave];
+
+    long lb = xcd_remap(blockIdx.x, gridDim.x);
+    long i = p.s_begin + lb * kWavesPerBlock + wave;
+    if (i >= p.s_end)
+        return;
+
+    Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+    Disc d;
+    disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+    double ga = 0.0, gb = 0.0;
+    if (OP != OP_FWD) {
+        ga = p.g_a ? (double)p.g_a[i] : 0.0;
+        gb = p.g_b ? (double)p.g_b[i] : 0.0;
+    }
+    double acc_a = 0.0, acc_b = 0.0;
+
+    for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+        int total = build_runs(p.hs, d, r0, lane, t);
+        int s = 0;
+        for (int v = lane; v < total; v += kWaveSize) {
+            while (v >= t.pref[s + 1])
+                s++;
+            int pix = t.pix0[s] + (v - t.pref[s]);
+            int r = t.ring[s];
+            int sp, nr;
+            bool sh;
+            ring_info(p.hs, r, sp, nr, sh);
+            double ra_pix = pixel_phi(p.hs, r, pix, sp);
+            double2 cs = p.ring_cs[r];
+            double chi;
+            int bp = beam_pixel(p.hb, c, ra_pix, cs.x, cs.y, chi);
+            if ((unsigned)bp < (unsigned)p.npix_beam) {   /* also rejects bp < 0 */
+                double s2d, c2d;
+                pl_sincos(2.0 * chi, &s2d, &c2d);
+                float c2 = (float)c2d, s2 = (float)s2d;
+                float4 A = p.beam[2 * (long)bp];
+                float4 B = p.beam[2 * (long)bp + 1];
+                if (OP == OP_FWD) {
+                    float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
+                    float ta = sI * (A.x) + sQ * (A.y * c2 - A.z * s2) +
+                               sU * (A.z * c2 + A.y * s2);
+                    float tb = sI * (B.x) + sQ * (-B.y * c2 + B.z * s2) +
+                               sU * (-B.z * c2 - B.y * s2);
+                    acc_a += ta;
+                    acc_b += tb;
+                } else if (OP == OP_ADJ_MAP) {
+                    double cc = c2, ss = s2;
+                    double vI = ga * A.x + gb * B.x;
+                    double vQ = ga * (A.y * cc - A.z * ss) + gb * (-B.y * cc + B.z * ss);
+                    double vU = ga * (A.z * cc + A.y * ss) + gb * (-B.z * cc - B.y * ss);
+                    unsafeAtomicAdd(&p.gI[pix], vI);
+                    unsafeAtomicAdd(&p.gQ[pix], vQ);
+                    unsafeAtomicAdd(&p.gU[pix], vU);
+                } else {
+                    double cc = c2, ss = s2;
+                    double q = p.Q[pix], u = p.U[pix], sI = p.I[pix];
+                    double *g = p.gbeam + 8 * (long)bp;
+                    unsafeAtomicAdd(&g[0], ga * sI);
+                    unsafeAtomicAdd(&g[1], ga * (q * cc + u * ss));
+                    unsafeAtomicAdd(&g[2], ga * (-q * ss + u * cc));
+                    unsafeAtomicAdd(&g[4], gb * sI);
+                    unsafeAtomicAdd(&g[5], gb * (-q * cc - u * ss));
+                    unsafeAtomicAdd(&g[6], gb * (q * ss - u * cc));
+                }
+            }
+        }
+        wave_sync();
+    }
+    if (OP == OP_FWD) {
+        acc_a = wave_sum(acc_a);
+        acc_b = wave_sum(acc_b);
+        if (lane == 0) {
+            if (p.tod_a)
+                p.tod_a[i] = (float)acc_a;
+            if (p.tod_b)
+                p.tod_b[i] = (float)acc_b;
+        }
+    }
+}
+
+/* cos and sin of the declination of every ring (dec = pi/2 - theta), indexed
+ * by ring number 1 .. 4N-1 (entry 0 unused). */
+__global__ void k_ring_table(Hpx h, double2 *out)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (r >= 4 * h.nside)
+        return;
+    double th = ring_theta(h, r);
+    double dec = kPi / 2.0 - th;
+    double s, c;
+    pl_sincos(dec, &s, &c);
+    out[r] = make_double2(c, s);
+}
+
+__global__ void k_fill_f64(double *x, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+__global__ void k_f64_to_f32(const double *x, float *y, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+/* [npix][8] doubles (A0..A3, B0..B3) -> eight float maps */
+__global__ void k_unpack_beam_grad(const double *g, long npix, float *a0, float *a1,
+                                   float *a2, float *a3, float *b0, float *b1,
+                                   float *b2, float *b3)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < npix; i += (long)gridDim.x * blockDim.x) {
+        const double *s = g + 8 * i;
+        a0[i] = (float)s[0]; a1[i] = (float)s[1]; a2[i] = (float)s[2]; a3[i] = (float)s[3];
+        b0[i] = (float)s[4]; b1[i] = (float)s[5]; b2[i] = (float)s[6]; b3[i] = (float)s[7];
+    }
+}
+
+/* eight float maps -> [npix][8] (float4 A, float4 B) */
+__global__ void k_pack_beam(long npix, const float *a0, const float *a1, const float *a2,
+                            const float *a3, const float *b0, const float *b1,
+                            const float *b2, const float *b3, float4 *out)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < npix; i += (long)gridDim.x * blockDim.x) {
+        out[2 * i] = make_float4(a0[i], a1[i], a2[i], a3 ? a3[i] : 0.f);
+        out[2 * i + 1] = make_float4(b0[i], b1[i], b2[i], b3 ? b3[i] : 0.f);
+    }
+}
+
+/* ---- test kernels: the HEALPix and matching primitives on their own ---- */
+
+__global__ void k_pix2ang(Hpx h, long n, const int *pix, double *th, double *ph)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        pix2ang(h, pix[i], th[i], ph[i]);
+}
+
+__global__ void k_ang2pix(Hpx h, long n, const double *th, const double *ph, int *pix)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        pix[i] = ang2pix(h, th[i], ph[i]);
+}
+
+__global__ void k_interpol(Hpx h, long n, const double *th, const double *ph, int *pix,
+                           double *wgt)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        get_interpol(h, th[i], ph[i], pix + 4 * i, wgt + 4 * i);
+}
+
+/* Disc of each pointing, one thread per pointing: count (pixels == NULL) or
+ * fill pixels at offsets[i]. */
+__global__ void k_disc(Hpx h, long n, const float *ra, const float *dec, double radius,
+                       double cosr, const long *offsets, int *counts, int *pixels)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    Centre c = centre_make(ra[i], dec[i], 0.f);
+    Disc d;
+    disc_setup(h, kHalfPi - c.dec, c.ra, radius, cosr, d);
+    long k = pixels ? offsets[i] : 0;
+    int cnt = 0;
+    for (int r = d.r_first; r <= d.r_last; r++) {
+        int p0[2], ln[2];
+        int ns = disc_ring(h, d, r, p0, ln);
+        for (int j = 0; j < ns; j++) {
+            if (pixels)
+                for (int q = 0; q < ln[j]; q++)
+                    pixels[k + cnt + q] = p0[j] + q;
+            cnt += ln[j];
+        }
+    }
+    if (counts)
+        counts[i] = cnt;
+}
+
+/* Sky pixel -> (beam pixel, chi) for pixel lists laid out as offsets/pixels
+ * (the quantity kernel 1 of the reference computes, cudaConv2.cu:59-128). */
+__global__ void k_match(Hpx hs, const double2 *ring_cs, Hpx hb, long n, const float *ra,
+                        const float *dec, const float *pa, const long *offsets,
+                        const int *pixels, int *bpix, double *chi)
+{
+    long i = (long)blockIdx.x;
+    if (i >= n)
+        return;
+    Centre c = centre_make(ra[i], dec[i], pa[i]);
+    for (long k = offsets[i] + threadIdx.x; k < offsets[i + 1]; k += blockDim.x) {
+        int pix = pixels[k];
+        int r = pix2ring(hs, pix);
+        int sp, nr;
+        bool sh;
+        ring_info(hs, r, sp, nr, sh);
+        double ra_pix = pixel_phi(hs, r, pix, sp);
+        double2 cs = ring_cs[r];
+        double x;
+        bpix[k] = beam_pixel(hb, c, ra_pix, cs.x, cs.y, x);
+        chi[k] = x;
+    }
+}
+
+/* Element-wise FP64 primitives, for checking host/device bit equality. */
+__global__ void k_libm(int op, long n, const double *x, const double *y, double *o)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x) {
+        double a = x[i], b = y ? y[i] : 0.0, r = 0.0;
+        switch (op) {
+        case 0: r = pl_sin(a); break;
+        case 1: r = pl_cos(a); break;
+        case 2: r = pl_acos(a); break;
+        case 3: r = pl_atan2(a, b); break;
+        case 4: r = sqrt(a); break;
+        case 5: r = a / b; break;
+        case 6: r = fmodulo(a, b); break;
+        case 7: { double s, c; pl_sincos(a, &s, &c); r = s + 2.0 * c; } break;
+        default: r = 0.0;
+        }
+        o[i] = r;
+    }
+}
+
+/* ---- launchers ---------------------------------------------------------- */
+
+static inline unsigned nblocks(long n, int per)
+{
+    long b = (n + per - 1) / per;
+    if (b < 1)
+        b = 1;
+    return (unsigned)b;
+}
+
+static inline unsigned gs_blocks(long n)
+{
+    long b = (n + 255) / 256;
+    if (b > 8192)
+        b = 8192;
+    if (b < 1)
+        b = 1;
+    return (unsigned)b;
+}
+
+hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
+{
+    long n = p.s_end - p.s_begin;
+    if (n <= 0)
+        return hipSuccess;
+    unsigned g = nblocks(n, kWavesPerBlock);
+    switch (op) {
+    case OP_FWD: hipLaunchKernelGGL(k_pair<OP_FWD>, dim3(g), dim3(kBlock), 0, st, p); break;
+    case OP_ADJ_MAP: hipLaunchKernelGGL(k_pair<OP_ADJ_MAP>, dim3(g), dim3(kBlock), 0, st, p); break;
+    case OP_ADJ_BEAM: hipLaunchKernelGGL(k_pair<OP_ADJ_BEAM>, dim3(g), dim3(kBlock), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ring_table(const Hpx &h, double2 *out, hipStream_t st)
+{
+    long n = 4L * h.nside - 1;
+    hipLaunchKernelGGL(k_ring_table, dim3(nblocks(n, 256)), dim3(256), 0, st, h, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f64(double *x, long n, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(gs_blocks(n)), dim3(256), 0, st, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(gs_blocks(n)), dim3(256), 0, st, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_beam_grad(const double *g, long npix, float *const out[8],
+                                   hipStream_t st)
+{
+    hipLaunchKernelGGL(k_unpack_beam_grad, dim3(gs_blocks(npix)), dim3(256), 0, st, g,
+                       npix, out[0], out[1], out[2], out[3], out[4], out[5], out[6],
+                       out[7]);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_beam(long npix, const float *const in[8], float4 *out,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_beam, dim3(gs_blocks(npix)), dim3(256), 0, st, npix, in[0],
+                       in[1], in[2], in[3], in[4], in[5], in[6], in[7], out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pix2ang(const Hpx &h, long n, const int *pix, double *th, double *ph,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pix2ang, dim3(gs_blocks(n)), dim3(256), 0, st, h, n, pix, th, ph);
+    return hipGetLastError();
+}
+
+hipError_t launch_ang2pix(const Hpx &h, long n, const double *th, const double *ph,
+                          int *pix, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ang2pix, dim3(gs_blocks(n)), dim3(256), 0, st, h, n, th, ph, pix);
+    return hipGetLastError();
+}
+
+hipError_t launch_interpol(const Hpx &h, long n, const double *th, const double *ph,
+                           int *pix, double *wgt, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_interpol, dim3(gs_blocks(n)), dim3(256), 0, st, h, n, th, ph, pix,
+                       wgt);
+    return hipGetLastError();
+}
+
+hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
+                       double radius, double cosr, const long *offsets, int *counts,
+                       int *pixels, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_disc, dim3(nblocks(n, 64)), dim3(64), 0, st, h, n, ra, dec, radius,
+                       cosr, offsets, counts, pixels);
+    return hipGetLastError();
+}
+
+hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb, long n,
+                        const float *ra, const float *dec, const float *pa,
+                        const long *offsets, const int *pixels, int *bpix, double *chi,
+                        hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, ring_cs, hb, n, ra,
+                       dec, pa, offsets, pixels, bpix, chi);
+    return hipGetLastError();
+}
+
+hipError_t launch_libm(int op, long n, const double *x, const double *y, double *o,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_libm, dim3(gs_blocks(n)), dim3(256), 0, st, op, n, x, y, o);
+    return hipGetLastError();
+}
+
+} /* namespace pisco */

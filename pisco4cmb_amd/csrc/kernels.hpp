@@ -1,0 +1,66 @@
+/*
+ * kernels.hpp -- internal interface between the C-ABI layer and the kernels.
+ */
+#ifndef PISCO_KERNELS_HPP
+#define PISCO_KERNELS_HPP
+
+#include <hip/hip_runtime.h>
+
+#include "device_hpx.h"
+
+namespace pisco {
+
+enum { OP_FWD = 0, OP_ADJ_MAP = 1, OP_ADJ_BEAM = 2 };
+
+/* One launch over samples [s_begin, s_end) of a detector pair a/b that
+ * share pointing and beam geometry. */
+struct PairParams {
+    /* sky: RING map of NSIDE hs.nside, SoA float I,Q,U */
+    Hpx hs;
+    const double2 *ring_cs;   /* (cos dec, sin dec) per ring, index 1..4N-1 */
+    const float *I, *Q, *U;
+    /* beam: first npix_beam RING pixels of a map of NSIDE hb.nside,
+     * AoS [npix_beam][2] float4 = (A0,A1,A2,A3), (B0,B1,B2,B3) */
+    Hpx hb;
+    int npix_beam;
+    double rho_max, cos_rmax;
+    const float4 *beam;
+    /* pointing (float radians; pa counter-clockwise as Scan holds it) */
+    const float *ra, *dec, *pa;
+    long s_begin, s_end;
+    /* forward output */
+    float *tod_a, *tod_b;
+    /* adjoint inputs (TOD cotangents) and outputs (FP64 accumulators) */
+    const float *g_a, *g_b;
+    double *gI, *gQ, *gU;     /* [npix sky] each */
+    double *gbeam;            /* [npix_beam][8] */
+};
+
+hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
+hipError_t launch_ring_table(const Hpx &h, double2 *out, hipStream_t st);
+hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
+hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
+hipError_t launch_unpack_beam_grad(const double *g, long npix, float *const out[8],
+                                   hipStream_t st);
+hipError_t launch_pack_beam(long npix, const float *const in[8], float4 *out,
+                            hipStream_t st);
+
+hipError_t launch_pix2ang(const Hpx &h, long n, const int *pix, double *th, double *ph,
+                          hipStream_t st);
+hipError_t launch_ang2pix(const Hpx &h, long n, const double *th, const double *ph,
+                          int *pix, hipStream_t st);
+hipError_t launch_interpol(const Hpx &h, long n, const double *th, const double *ph,
+                           int *pix, double *wgt, hipStream_t st);
+hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
+                       double radius, double cosr, const long *offsets, int *counts,
+                       int *pixels, hipStream_t st);
+hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb, long n,
+                        const float *ra, const float *dec, const float *pa,
+                        const long *offsets, const int *pixels, int *bpix, double *chi,
+                        hipStream_t st);
+hipError_t launch_libm(int op, long n, const double *x, const double *y, double *o,
+                       hipStream_t st);
+
+} /* namespace pisco */
+
+#endif

@@ -1,0 +1,121 @@
+"""torch.autograd binding of the convolution (device tensors, torch's stream).
+
+    tod_a, tod_b = beam_convolve(I, Q, U, beam_a, beam_b, ra, dec, pa,
+                                 nside_sky, nside_beam)
+
+I, Q, U: float32 [12*nside_sky^2] cuda tensors; beam_a, beam_b: float32
+[4, npix_beam] (PolBeam aBeams / bBeams); ra, dec, pa: float32 [nsamples].
+The forward is the CPU Convolver's map (convolver.cpp:74-183) computed by the
+HIP kernels.  backward() runs the hand-written adjoint kernels: the sky
+gradient (TOD -> I,Q,U scatter) and, when requested, the beam gradient; the
+pixel lists and beam-frame indices are recomputed in backward, never stored.
+Pointing gets no gradient (the map is not differentiable in it).
+
+torch only supplies device memory and the current stream; every computation
+is a libpisco kernel.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr_array
+
+_CTX = {}
+
+
+def _ctx(device):
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx not in _CTX:
+        c = ctypes.c_void_p()
+        cfg = _lib.RunConfig(nStreams=1, maxMemUsage=1 << 48, deviceId=idx,
+                             ptgPerConv=1 << 22, pixelsPerDisc=0)
+        check(lib.pisco_create(ctypes.byref(cfg), ctypes.byref(c)))
+        _CTX[idx] = c
+    return _CTX[idx]
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _req(t, name, dtype=torch.float32):
+    if not t.is_cuda or t.dtype != dtype:
+        raise ValueError("%s must be a %s cuda tensor" % (name, dtype))
+    return t.contiguous()
+
+
+def pack_beam(beam_a, beam_b):
+    """[4,npix] x 2 -> [npix, 8] device layout used by the kernels."""
+    beam_a = _req(beam_a, "beam_a")
+    beam_b = _req(beam_b, "beam_b")
+    npix = beam_a.shape[1]
+    out = torch.empty((npix, 8), dtype=torch.float32, device=beam_a.device)
+    maps = ptr_array([beam_a[k].data_ptr() for k in range(4)] +
+                     [beam_b[k].data_ptr() for k in range(4)])
+    check(lib.pisco_dev_pack_beam(_ctx(beam_a.device), _stream(beam_a.device), npix, maps,
+                                  out.data_ptr()))
+    return out
+
+
+class BeamConvolution(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, I, Q, U, beam_a, beam_b, ra, dec, pa, nside_sky, nside_beam):
+        dev = I.device
+        I, Q, U = (_req(x, n) for x, n in ((I, "I"), (Q, "Q"), (U, "U")))
+        ra, dec, pa = (_req(x, n) for x, n in ((ra, "ra"), (dec, "dec"), (pa, "pa")))
+        npix_sky = 12 * nside_sky * nside_sky
+        if I.numel() != npix_sky or Q.numel() != npix_sky or U.numel() != npix_sky:
+            raise ValueError("sky maps must have 12*nside_sky^2 pixels")
+        n = ra.numel()
+        if dec.numel() != n or pa.numel() != n:
+            raise ValueError("ra, dec, pa must have the same length")
+        beam8 = pack_beam(beam_a, beam_b)
+        npix_beam = beam8.shape[0]
+        tod_a = torch.empty(n, dtype=torch.float32, device=dev)
+        tod_b = torch.empty(n, dtype=torch.float32, device=dev)
+        check(lib.pisco_dev_forward(_ctx(dev), _stream(dev), nside_sky, I.data_ptr(),
+                                    Q.data_ptr(), U.data_ptr(), nside_beam, npix_beam,
+                                    beam8.data_ptr(), n, ra.data_ptr(), dec.data_ptr(),
+                                    pa.data_ptr(), tod_a.data_ptr(), tod_b.data_ptr()))
+        ctx.save_for_backward(I, Q, U, beam8, ra, dec, pa)
+        ctx.nside_sky = nside_sky
+        ctx.nside_beam = nside_beam
+        return tod_a, tod_b
+
+    @staticmethod
+    def backward(ctx, g_a, g_b):
+        I, Q, U, beam8, ra, dec, pa = ctx.saved_tensors
+        dev = I.device
+        n = ra.numel()
+        npix_sky = I.numel()
+        npix_beam = beam8.shape[0]
+        ga = g_a.contiguous().float() if g_a is not None else None
+        gb = g_b.contiguous().float() if g_b is not None else None
+        gI = gQ = gU = gbeam_a = gbeam_b = None
+        if any(ctx.needs_input_grad[:3]):
+            gI = torch.empty(npix_sky, dtype=torch.float32, device=dev)
+            gQ = torch.empty_like(gI)
+            gU = torch.empty_like(gI)
+            check(lib.pisco_dev_adjoint_map(
+                _ctx(dev), _stream(dev), ctx.nside_sky, ctx.nside_beam, npix_beam,
+                beam8.data_ptr(), n, ra.data_ptr(), dec.data_ptr(), pa.data_ptr(),
+                ga.data_ptr() if ga is not None else None,
+                gb.data_ptr() if gb is not None else None,
+                gI.data_ptr(), gQ.data_ptr(), gU.data_ptr()))
+        if ctx.needs_input_grad[3] or ctx.needs_input_grad[4]:
+            g8 = torch.empty((npix_beam, 8), dtype=torch.float32, device=dev)
+            check(lib.pisco_dev_adjoint_beam(
+                _ctx(dev), _stream(dev), ctx.nside_sky, I.data_ptr(), Q.data_ptr(),
+                U.data_ptr(), ctx.nside_beam, npix_beam, n, ra.data_ptr(), dec.data_ptr(),
+                pa.data_ptr(), ga.data_ptr() if ga is not None else None,
+                gb.data_ptr() if gb is not None else None, g8.data_ptr()))
+            gbeam_a = g8[:, :4].t().contiguous()
+            gbeam_b = g8[:, 4:].t().contiguous()
+        return gI, gQ, gU, gbeam_a, gbeam_b, None, None, None, None, None
+
+
+def beam_convolve(I, Q, U, beam_a, beam_b, ra, dec, pa, nside_sky, nside_beam):
+    """Differentiable (tod_a, tod_b) of a detector pair."""
+    return BeamConvolution.apply(I, Q, U, beam_a, beam_b, ra, dec, pa, nside_sky,
+                                 nside_beam)
