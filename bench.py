@@ -1,0 +1,201 @@
+"""Benchmark of the pixel-space beam convolution on one MI355X.
+
+Default workload (the headline metric): forward + adjoint (TOD -> I,Q,U map)
+of a detector pair at sky NSIDE 512 with a 5 degree beam cap:
+  sky    NSIDE 512 RING, float32 I,Q,U power-law Gaussian random fields
+         (C_l ~ l^-2, l <= 512), seeded
+  beams  elliptical-Gaussian pair, FWHM 0.33 deg, cap 5 deg, beam NSIDE 4096
+         (382,812 beam pixels per detector)
+  scan   4e6 samples of a constant-elevation whole-sky scan (lat -23 deg,
+         el 45 deg, 1 rpm, 100 Hz, sidereal drift; dec in [-68, +22] deg)
+A step is one forward over all samples followed by one adjoint-map over all
+samples (cotangents = a fixed random vector), inputs resident in HBM.
+
+Roofline (gather model): a visit is one (sample, disc pixel) pair and reads
+the sky I,Q,U (12 B) and both beams' I,Q,U (24 B); per sample 12 B of
+pointing plus 4 B per detector of TOD (written by the forward, read as the
+cotangent by the adjoint); the adjoint also reads and writes the 12 B I,Q,U
+gradient of every visit:
+  B_fwd  = sum_i |D_i| * 36 + N * 20,   B_adjm = sum_i |D_i| * 48 + N * 20
+with |D_i| the exact disc sizes (device query_disc).  achieved = bytes of
+the dominant kernel / its mean duration (HIP events on the launch stream).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
+                  [--no-cpu] [--forms] [--profile-steps K]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12   # bytes/s, MI355X
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--samples", type=int, default=4_000_000)
+    p.add_argument("--cpu-samples", type=int, default=20_000)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--forms", action="store_true",
+                   help="also time the plain drop-in forward (config 2)")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+              "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from pisco4cmb_amd import inputs, torch_ops
+    from pisco4cmb_amd._lib import check, lib
+
+    dev = torch.device("cuda:0")
+    nside_sky, nside_beam, cap_deg, fwhm = 512, 4096, 5.0, 0.33
+    n = args.samples
+
+    t0 = time.time()
+    I, Q, U, _ = inputs.powerlaw_sky(nside_sky, seed=2024, lmax=512)
+    beam = inputs.gaussian_polbeam(nside_beam, cap_deg, fwhm, 0.0, 0.0, nside_sky=nside_sky)
+    ra, dec, pa = inputs.ces_scan(n)
+    g = np.random.default_rng(7).normal(size=(2, n)).astype(np.float32)
+    setup_s = time.time() - t0
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    It, Qt, Ut = t(I), t(Q), t(U)
+    ba, bb = t(np.stack(beam.aBeams)), t(np.stack(beam.bBeams))
+    rat, dect, pat = t(ra), t(dec), t(pa)
+    gat, gbt = t(g[0]), t(g[1])
+    beam8 = torch_ops.pack_beam(ba, bb)
+    npb = beam8.shape[0]
+    npix_sky = 12 * nside_sky * nside_sky
+    tod_a = torch.empty(n, dtype=torch.float32, device=dev)
+    tod_b = torch.empty_like(tod_a)
+    gI = torch.empty(npix_sky, dtype=torch.float32, device=dev)
+    gQ = torch.empty_like(gI)
+    gU = torch.empty_like(gI)
+    ctx = torch_ops._ctx(dev)
+    stream = torch_ops._stream(dev)
+
+    # exact disc sizes (device query_disc, the same code the kernels run)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    rho_max = lib.pisco_beam_rho_max(nside_beam, npb)
+    check(lib.pisco_dev_query_disc(ctx, stream, nside_sky, n, rat.data_ptr(), dect.data_ptr(),
+                                   rho_max, None, counts.data_ptr(), None))
+    torch.cuda.synchronize()
+    visits = int(counts.to(torch.int64).sum().item())
+    k = 2
+    b_fwd = visits * (12 + 12 * k) + n * (12 + 4 * k)
+    b_adj = visits * (12 * k + 24) + n * (12 + 4 * k)
+
+    def fwd():
+        check(lib.pisco_dev_forward(ctx, stream, nside_sky, It.data_ptr(), Qt.data_ptr(),
+                                    Ut.data_ptr(), nside_beam, npb, beam8.data_ptr(), n,
+                                    rat.data_ptr(), dect.data_ptr(), pat.data_ptr(),
+                                    tod_a.data_ptr(), tod_b.data_ptr()))
+
+    def adj():
+        check(lib.pisco_dev_adjoint_map(ctx, stream, nside_sky, nside_beam, npb,
+                                        beam8.data_ptr(), n, rat.data_ptr(), dect.data_ptr(),
+                                        pat.data_ptr(), gat.data_ptr(), gbt.data_ptr(),
+                                        gI.data_ptr(), gQ.data_ptr(), gU.data_ptr()))
+
+    for _ in range(args.warmup):
+        fwd()
+        adj()
+    torch.cuda.synchronize()
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.time()
+    start.record()
+    for e0, e1, e2 in ev:
+        e0.record()
+        fwd()
+        e1.record()
+        adj()
+        e2.record()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.time() - wall0
+    total_ms = start.elapsed_time(stop)
+    t_fwd = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps / 1e3
+    t_adj = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps / 1e3
+    ms_per_step = total_ms / args.steps
+    value = k * n / (ms_per_step / 1e3)
+
+    dominant = "adjoint_map" if t_adj >= t_fwd else "forward"
+    b_dom, t_dom = (b_adj, t_adj) if dominant == "adjoint_map" else (b_fwd, t_fwd)
+    achieved = b_dom / t_dom
+    out = {
+        "metric": "TOD samples/s fwd+adjoint @NSIDE512 5deg beam (pair); %HBM peak",
+        "value": value,
+        "unit": "TOD samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 data/products, f64 index path + accumulation",
+        "data": "synthetic (seeded power-law sky, Gaussian beams, CES scan)",
+        "config": {"workload": "fwd+adjoint-map, sky NSIDE 512, pair, FWHM 0.33 deg, cap "
+                               "5 deg, beam NSIDE 4096, %d samples whole-sky CES" % n,
+                   "nside_sky": nside_sky, "nside_beam": nside_beam, "npix_beam": npb,
+                   "samples": n, "detectors": k, "visits_per_pass": visits,
+                   "mean_disc": visits / n},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_pair<%s>" % ("OP_ADJ_MAP" if dominant == "adjoint_map"
+                                               else "OP_FWD"),
+                     "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
+        "fwd_adj": {"forward_s": t_fwd, "adjoint_s": t_adj,
+                    "frac_hbm_fwd_plus_adj": (b_fwd + b_adj) / (t_fwd + t_adj) / HBM_PEAK,
+                    "forward_tod_samples_per_s": k * n / t_fwd,
+                    "wall_s_timed": wall, "setup_s": setup_s},
+    }
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra,
+                                           dec, pa, g)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g):
+    """The CPU oracle (a C restatement of the reference Convolver, OpenMP over
+    samples) on a prefix of the same workload: forward + adjoint-map."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind
+    orc = oracle_bind.Oracle()
+    m = min(args.cpu_samples, ra.size)
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+    beams = list(beam.aBeams) + list(beam.bBeams)
+    t0 = time.time()
+    orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:m], dec[:m], pa[:m], nthreads=cores)
+    t1 = time.time()
+    orc.adjoint_map(nside_sky, nside_beam, beams, ra[:m], dec[:m], pa[:m], g[0, :m], g[1, :m])
+    t2 = time.time()
+    return {"value": 2 * m / (t2 - t0), "unit": "TOD samples/s", "cores": cores,
+            "kind": "port",
+            "sample": "first %d samples of the same workload, forward (%.1f s) + "
+                      "adjoint-map (%.1f s), OpenMP over samples" % (m, t1 - t0, t2 - t1)}
+
+
+if __name__ == "__main__":
+    main()

@@ -28,8 +28,9 @@
  *
  * Host-buffer entry points copy their inputs; callers may free them after the
  * call returns.  pisco_dev_* entry points take device pointers and a
- * hipStream_t (NULL = the context's own stream) and are asynchronous with
- * respect to the host.
+ * hipStream_t (NULL = HIP's null stream) and are asynchronous with respect to
+ * the host.  Calls on one context share its workspace and must not overlap
+ * in time on different streams.
  */
 #ifndef PISCO_H
 #define PISCO_H
@@ -166,6 +167,13 @@ int pisco_dev_match(pisco_ctx *ctx, void *stream, int nside_sky,
                     int nside_beam, long n, const float *ra, const float *dec,
                     const float *pa, const long *offsets, const int *pixels,
                     int *bpix, double *chi);
+/* As pisco_dev_match; mode 1 evaluates the kernels' certified fast form
+ * (same beam pixels by construction) and writes float cos(2chi), sin(2chi)
+ * packed into each 8-byte chi element instead of chi. */
+int pisco_dev_match_mode(pisco_ctx *ctx, void *stream, int mode, int nside_sky,
+                         int nside_beam, long n, const float *ra,
+                         const float *dec, const float *pa, const long *offsets,
+                         const int *pixels, int *bpix, double *chi);
 /* Elementwise FP64 primitives of the index path on the device (op: 0 sin,
  * 1 cos, 2 acos, 3 atan2(x,y), 4 sqrt, 5 x/y, 6 fmodulo(x,y), 7 sincos). */
 int pisco_dev_libm(pisco_ctx *ctx, void *stream, int op, long n,

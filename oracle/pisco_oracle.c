@@ -712,21 +712,34 @@ static void o_sample(const conv_t *cv, double ra_bc, double dec_bc,
                               (double)sU * (-bb2 * c2d - bb1 * s2d);
                 } else if (kind == V_ADJ_MAP) {
                     double c = c2chi, s = s2chi;
-                    gmap[3 * (long)skyPix + 0] += ga * ba0 + gb * bb0;
-                    gmap[3 * (long)skyPix + 1] += ga * (ba1 * c - ba2 * s) +
-                                                  gb * (-bb1 * c + bb2 * s);
-                    gmap[3 * (long)skyPix + 2] += ga * (ba2 * c + ba1 * s) +
-                                                  gb * (-bb2 * c - bb1 * s);
+                    double vI = ga * ba0 + gb * bb0;
+                    double vQ = ga * (ba1 * c - ba2 * s) + gb * (-bb1 * c + bb2 * s);
+                    double vU = ga * (ba2 * c + ba1 * s) + gb * (-bb2 * c - bb1 * s);
+                    double *g = gmap + 3 * (long)skyPix;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    g[0] += vI;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    g[1] += vQ;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    g[2] += vU;
                 } else {
                     double c = c2chi, s = s2chi;
                     double q = sQ, u = sU;
+                    double v[6] = {ga * sI, ga * (q * c + u * s), ga * (-q * s + u * c),
+                                   gb * sI, gb * (-q * c - u * s), gb * (q * s - u * c)};
                     double *g = gbeam + 6 * (long)bpix;
-                    g[0] += ga * sI;
-                    g[1] += ga * (q * c + u * s);
-                    g[2] += ga * (-q * s + u * c);
-                    g[3] += gb * sI;
-                    g[4] += gb * (-q * c - u * s);
-                    g[5] += gb * (q * s - u * c);
+                    for (int k = 0; k < 6; k++) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                        g[k] += v[k];
+                    }
                 }
             }
         }
@@ -807,21 +820,29 @@ void or_adjoint_map(int nside_sky, int nside_beam, long npix_beam,
 {
     conv_t cv = o_conv_make(nside_sky, NULL, NULL, NULL, nside_beam, npix_beam,
                             beams);
-    rset_t rs = {0, 0, 0};
     memset(gmap, 0, sizeof(double) * 3 * (size_t)cv.hs.npix);
     /* o_sample loads sky values for every visit; the map adjoint does not
      * use them, so it is given an all-zero sky. */
     float *zs = (float *)calloc((size_t)cv.hs.npix, sizeof(float));
     cv.I = zs; cv.Q = zs; cv.U = zs;
-    for (long i = 0; i < nsamples; i++) {
-        float ra_bc = ra[i];
-        double dec_bc = dec[i];
-        double pa_bc = -pa[i];
-        o_sample(&cv, ra_bc, dec_bc, pa_bc, V_ADJ_MAP, 0, 0,
-                 g_a ? g_a[i] : 0.0, g_b ? g_b[i] : 0.0, gmap, 0, &rs);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        rset_t rt = {0, 0, 0};
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (long i = 0; i < nsamples; i++) {
+            float ra_bc = ra[i];
+            double dec_bc = dec[i];
+            double pa_bc = -pa[i];
+            o_sample(&cv, ra_bc, dec_bc, pa_bc, V_ADJ_MAP, 0, 0,
+                     g_a ? g_a[i] : 0.0, g_b ? g_b[i] : 0.0, gmap, 0, &rt);
+        }
+        free(rt.r);
     }
     free(zs);
-    free(rs.r);
 }
 
 /* Adjoint with respect to the beam maps, in FP64: gbeam[6*b + k] for
@@ -834,16 +855,24 @@ void or_adjoint_beam(int nside_sky, const float *I, const float *Q,
                      const float *g_b, double *gbeam)
 {
     conv_t cv = o_conv_make(nside_sky, I, Q, U, nside_beam, npix_beam, beams);
-    rset_t rs = {0, 0, 0};
     memset(gbeam, 0, sizeof(double) * 6 * (size_t)npix_beam);
-    for (long i = 0; i < nsamples; i++) {
-        float ra_bc = ra[i];
-        double dec_bc = dec[i];
-        double pa_bc = -pa[i];
-        o_sample(&cv, ra_bc, dec_bc, pa_bc, V_ADJ_BEAM, 0, 0,
-                 g_a ? g_a[i] : 0.0, g_b ? g_b[i] : 0.0, 0, gbeam, &rs);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        rset_t rt = {0, 0, 0};
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (long i = 0; i < nsamples; i++) {
+            float ra_bc = ra[i];
+            double dec_bc = dec[i];
+            double pa_bc = -pa[i];
+            o_sample(&cv, ra_bc, dec_bc, pa_bc, V_ADJ_BEAM, 0, 0,
+                     g_a ? g_a[i] : 0.0, g_b ? g_b[i] : 0.0, 0, gbeam, &rt);
+        }
+        free(rt.r);
     }
-    free(rs.r);
 }
 
 /* Number of disc pixels per sample (exact |D_i|), for the byte model. */

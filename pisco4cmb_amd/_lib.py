@@ -64,6 +64,8 @@ SIGNATURES = {
                                      P, P]),
     "pisco_dev_match": (c_int, [c_void_p, c_void_p, c_int, c_int, c_long, P, P, P, P, P,
                                 P, P]),
+    "pisco_dev_match_mode": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_long, P, P,
+                                     P, P, P, P, P]),
     "pisco_dev_libm": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_beam_rho_max": (c_double, [c_int, c_long]),
     "pisco_ang2pix": (c_long, [c_int, c_double, c_double]),

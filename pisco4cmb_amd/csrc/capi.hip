@@ -106,7 +106,13 @@ struct pisco_ctx {
 
 namespace {
 
-hipStream_t pick(pisco_ctx *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+/* Stream of a pisco_dev_* call: the caller's hipStream_t; NULL is HIP's
+ * null (legacy default) stream, which is also torch's default stream. */
+hipStream_t pick(pisco_ctx *c, void *s)
+{
+    (void)c;
+    return (hipStream_t)s;
+}
 
 int ring_table(pisco_ctx *c, int nside, const double2 **out)
 {
@@ -162,6 +168,7 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
     p.npix_beam = (int)npix_beam;
     p.rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
     p.cos_rmax = cos_radius(p.rho_max);
+    p.fc = fast_consts(p.hb);
     p.beam = (const float4 *)beam8;
     p.ra = ra;
     p.dec = dec;
@@ -713,6 +720,14 @@ int pisco_dev_match(pisco_ctx *c, void *stream, int nside_sky, int nside_beam, l
                     const float *ra, const float *dec, const float *pa, const long *offsets,
                     const int *pixels, int *bpix, double *chi)
 {
+    return pisco_dev_match_mode(c, stream, 0, nside_sky, nside_beam, n, ra, dec, pa, offsets,
+                                pixels, bpix, chi);
+}
+
+int pisco_dev_match_mode(pisco_ctx *c, void *stream, int mode, int nside_sky, int nside_beam,
+                         long n, const float *ra, const float *dec, const float *pa,
+                         const long *offsets, const int *pixels, int *bpix, double *chi)
+{
     if (!c || !valid_nside(nside_sky) || !valid_nside(nside_beam) || n < 0)
         return fail(PISCO_ERR_INVALID, "bad argument");
     if (n == 0)
@@ -720,7 +735,9 @@ int pisco_dev_match(pisco_ctx *c, void *stream, int nside_sky, int nside_beam, l
     HIPCHK(hipSetDevice(c->device));
     const double2 *rt;
     CHK(ring_table(c, nside_sky, &rt));
-    HIPCHK(launch_match(hpx_make(nside_sky), rt, hpx_make(nside_beam), n, ra, dec, pa,
+    Hpx hb = hpx_make(nside_beam);
+    FastConsts fc = fast_consts(hb);
+    HIPCHK(launch_match(hpx_make(nside_sky), rt, hb, mode ? &fc : nullptr, n, ra, dec, pa,
                         offsets, pixels, bpix, chi, pick(c, stream)));
     return PISCO_OK;
 }

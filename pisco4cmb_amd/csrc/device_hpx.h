@@ -382,6 +382,117 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
     return ang2pix(hb, rho, sigma);
 }
 
+/* ---- certified fast form of the same mapping ----------------------------- */
+
+/* The CPU path obtains the beam pixel from rho = acos(crho) through
+ * z = cos(rho) (and sin(rho) within 0.01 rad of the pole); in the polar branch
+ * of ang2pix the two floors that depend on rho are jp = (int)(tp*tmp) and
+ * jm = (int)((1-tp)*tmp) with tmp = nside*sqrt(3*(1-z)) (or, within 0.01 rad,
+ * nside*sin(rho)/sqrt((1+z)/3)).  Both forms equal nside*sqrt(3*(1-crho))
+ * up to the rounding of acos/cos/sin, which is bounded by
+ *     |tmp_exact - tmp_fast| <= nside*4.4e-16/sqrt(1-crho) + 2e-15*tmp
+ * (pisco_libm's acos, cos and sin are within 2 ulp).  beam_pixel_fast
+ * evaluates the cheap form and accepts its floors only if both fractional
+ * parts are farther than 100 times that bound (plus 1e-10) from an integer;
+ * otherwise it evaluates the exact path.  Either way the returned pixel is
+ * the one beam_pixel returns.  sigma (hence tt, tp and ip) is computed with
+ * the same operations as the exact path.
+ *
+ * chi is needed only as float cos(2 chi), sin(2 chi).  With
+ * beta = atan2(yb, xb), alpha = atan2(ya, xa) and chi = beta + alpha + psi - pi
+ * (mod 2 pi; sphtrigo.cpp:41-53), e^{2 i chi} = w^2/|w|^2 * e^{2 i psi} with
+ * w = (xb + i yb)(xa + i ya): no transcendental is needed (FP64 algebra,
+ * within ~1e-15 of cos/sin(2 chi) before rounding to float). */
+struct FastConsts {
+    double crho_big;    /* crho <  crho_big  => rho > 1e-6 for sure */
+    double crho_small;  /* crho >= crho_small => rho <= 1e-6 for sure */
+    float nside_marg;   /* 100 * 4.4e-16 * nside */
+};
+
+/* Rare paths of beam_pixel_fast, kept out of line so that they do not
+ * occupy registers in the hot loop. */
+__device__ __noinline__ int beam_pixel_exact_rho(const Hpx &hb, double crho, double sigma)
+{
+    double rho = (crho >= 1.0) ? 0.0 : pl_acos(crho);
+    return ang2pix(hb, rho, sigma);
+}
+
+__device__ __noinline__ bool rho_above_1e6(double crho)
+{
+    return (crho >= 1.0) ? false : (pl_acos(crho) > 1.0e-6);
+}
+
+__device__ __noinline__ double2 e2i_angles(double yb, double xb, double alpha)
+{
+    double beta = pl_atan2(yb, xb);
+    double s2c, c2c;
+    pl_sincos(2.0 * (beta + alpha), &s2c, &c2c);
+    return make_double2(c2c, s2c);
+}
+
+__device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const Centre &c,
+                                      double c2psi, double s2psi, double ra_pix,
+                                      double cdp, double sdp, float &c2, float &s2)
+{
+    double sd, cd;
+    pl_sincos(ra_pix - c.ra, &sd, &cd);
+    double crho = c.sdc * sdp + c.cdc * cdp * cd;
+    bool big;   /* rho > 1e-6 (sphtrigo.cpp:40) */
+    if (crho < fc.crho_big)
+        big = true;
+    else if (crho >= fc.crho_small)
+        big = false;
+    else
+        big = rho_above_1e6(crho);
+    double yb = sd * c.cdc, xb = c.sdc * cdp - c.cdc * sdp * cd;
+    double ya = sd * cdp, xa = sdp * c.cdc - cdp * c.sdc * cd;
+    double alpha = big ? pl_atan2(ya, xa) : kHalfPi;
+    double sigma = kPi - alpha - c.psi;
+    if (sigma < 0.0)
+        sigma += 2.0 * kPi;
+    else if (sigma > 2.0 * kPi)
+        sigma -= 2.0 * kPi;
+
+    /* cos/sin 2chi */
+    double er = 1.0, ei = 0.0;   /* e^{2i(alpha+beta)}; alpha = beta = pi/2 -> 1 */
+    if (big) {
+        double wr = xb * xa - yb * ya, wi = xb * ya + yb * xa;
+        double n = wr * wr + wi * wi;
+        if (n > 1e-280) {
+            double inv = 1.0 / n;
+            er = (wr * wr - wi * wi) * inv;
+            ei = 2.0 * wr * wi * inv;
+        } else {   /* degenerate geometry: take the angles themselves */
+            double2 e = e2i_angles(yb, xb, alpha);
+            er = e.x;
+            ei = e.y;
+        }
+    }
+    c2 = (float)(er * c2psi - ei * s2psi);
+    s2 = (float)(er * s2psi + ei * c2psi);
+
+    /* beam pixel: polar branch of loc2pix with the certified floors */
+    double omc = 1.0 - crho;
+    omc = omc > 0.0 ? omc : 0.0;
+    double tmp = hb.nside * sqrt(3 * omc);
+    double tt = fmodulo(sigma * kInvHalfPi, 4.0);
+    double tp = tt - (int)tt;
+    double u1 = tp * tmp;
+    double u2 = (1.0 - tp) * tmp;
+    int jp = (int)u1, jm = (int)u2;
+    double d1 = u1 - jp, d2 = u2 - jm;
+    float m = fc.nside_marg * rsqrtf((float)omc) + 2e-13f * (float)tmp + 1e-10f;
+    double md = (double)m;
+    bool ok = (crho > kTwoThird + 1e-12) && (d1 > md) && (d1 < 1.0 - md) && (d2 > md) &&
+              (d2 < 1.0 - md);
+    if (ok) {
+        int ir = jp + jm + 1;
+        int ip = (int)(tt * ir);
+        return 2 * ir * (ir - 1) + ip;
+    }
+    return beam_pixel_exact_rho(hb, crho, sigma);
+}
+
 /* get_interpol (Healpix_Base::get_interpol; the device port in the reference
  * is cudaHealpix.cu:198-259). */
 __host__ __device__ inline void ring_info2(const Hpx &h, int ring, int &startpix,

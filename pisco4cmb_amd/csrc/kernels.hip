@@ -35,6 +35,16 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* A wave-uniform double moved to scalar registers (one wave = one sample,
+ * so the sample's constants need no vector registers). */
+__device__ inline double uni(double x)
+{
+    long long v = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffLL));
+    int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ inline int wave_incl_scan(int x, int lane)
 {
 #pragma unroll
@@ -105,7 +115,7 @@ template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
 {
     __shared__ RunTable tables[kWavesPerBlock];
-    const int wave = threadIdx.x / kWaveSize;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
     const int lane = threadIdx.x % kWaveSize;
     RunTable &t = tables[wave];
 
@@ -115,12 +125,21 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
         return;
 
     Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+    c.ra = uni(c.ra);
+    c.dec = uni(c.dec);
+    c.psi = uni(c.psi);
+    c.cdc = uni(c.cdc);
+    c.sdc = uni(c.sdc);
+    double s2psi, c2psi;
+    pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+    s2psi = uni(s2psi);
+    c2psi = uni(c2psi);
     Disc d;
     disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
     double ga = 0.0, gb = 0.0;
     if (OP != OP_FWD) {
-        ga = p.g_a ? (double)p.g_a[i] : 0.0;
-        gb = p.g_b ? (double)p.g_b[i] : 0.0;
+        ga = uni(p.g_a ? (double)p.g_a[i] : 0.0);
+        gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
     }
     double acc_a = 0.0, acc_b = 0.0;
 
@@ -137,12 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
             ring_info(p.hs, r, sp, nr, sh);
             double ra_pix = pixel_phi(p.hs, r, pix, sp);
             double2 cs = p.ring_cs[r];
-            double chi;
-            int bp = beam_pixel(p.hb, c, ra_pix, cs.x, cs.y, chi);
+            float c2, s2;
+            int bp = beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, cs.x, cs.y, c2, s2);
             if ((unsigned)bp < (unsigned)p.npix_beam) {   /* also rejects bp < 0 */
-                double s2d, c2d;
-                pl_sincos(2.0 * chi, &s2d, &c2d);
-                float c2 = (float)c2d, s2 = (float)s2d;
                 float4 A = p.beam[2 * (long)bp];
                 float4 B = p.beam[2 * (long)bp + 1];
                 if (OP == OP_FWD) {
@@ -294,14 +310,18 @@ __global__ void k_disc(Hpx h, long n, const float *ra, const float *dec, double 
 
 /* Sky pixel -> (beam pixel, chi) for pixel lists laid out as offsets/pixels
  * (the quantity kernel 1 of the reference computes, cudaConv2.cu:59-128). */
-__global__ void k_match(Hpx hs, const double2 *ring_cs, Hpx hb, long n, const float *ra,
-                        const float *dec, const float *pa, const long *offsets,
-                        const int *pixels, int *bpix, double *chi)
+/* fast != 0: the kernels' certified form (beam_pixel_fast); chi[k] then
+ * receives float cos(2chi) and sin(2chi) packed as two floats. */
+__global__ void k_match(Hpx hs, const double2 *ring_cs, Hpx hb, FastConsts fc, int fast,
+                        long n, const float *ra, const float *dec, const float *pa,
+                        const long *offsets, const int *pixels, int *bpix, double *chi)
 {
     long i = (long)blockIdx.x;
     if (i >= n)
         return;
     Centre c = centre_make(ra[i], dec[i], pa[i]);
+    double s2psi, c2psi;
+    pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
     for (long k = offsets[i] + threadIdx.x; k < offsets[i + 1]; k += blockDim.x) {
         int pix = pixels[k];
         int r = pix2ring(hs, pix);
@@ -310,9 +330,16 @@ __global__ void k_match(Hpx hs, const double2 *ring_cs, Hpx hb, long n, const fl
         ring_info(hs, r, sp, nr, sh);
         double ra_pix = pixel_phi(hs, r, pix, sp);
         double2 cs = ring_cs[r];
-        double x;
-        bpix[k] = beam_pixel(hb, c, ra_pix, cs.x, cs.y, x);
-        chi[k] = x;
+        if (fast) {
+            float c2, s2;
+            bpix[k] = beam_pixel_fast(hb, fc, c, c2psi, s2psi, ra_pix, cs.x, cs.y, c2, s2);
+            float2 *o = (float2 *)&chi[k];
+            *o = make_float2(c2, s2);
+        } else {
+            double x;
+            bpix[k] = beam_pixel(hb, c, ra_pix, cs.x, cs.y, x);
+            chi[k] = x;
+        }
     }
 }
 
@@ -443,16 +470,28 @@ hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
     return hipGetLastError();
 }
 
-hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb, long n,
-                        const float *ra, const float *dec, const float *pa,
-                        const long *offsets, const int *pixels, int *bpix, double *chi,
-                        hipStream_t st)
+hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb,
+                        const FastConsts *fc, long n, const float *ra, const float *dec,
+                        const float *pa, const long *offsets, const int *pixels, int *bpix,
+                        double *chi, hipStream_t st)
 {
     if (n <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, ring_cs, hb, n, ra,
-                       dec, pa, offsets, pixels, bpix, chi);
+    FastConsts f = fc ? *fc : fast_consts(hb);
+    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, ring_cs, hb, f,
+                       fc ? 1 : 0, n, ra, dec, pa, offsets, pixels, bpix, chi);
     return hipGetLastError();
+}
+
+/* Thresholds of the rho > 1e-6 test on crho (host, with the same libm):
+ * rho >= 1.5e-6 is surely above 1e-6 and rho <= 0.5e-6 surely below. */
+FastConsts fast_consts(const Hpx &hb)
+{
+    FastConsts f;
+    f.crho_big = pl_cos(1.5e-6);
+    f.crho_small = pl_cos(0.5e-6);
+    f.nside_marg = (float)(100.0 * 4.4e-16 * hb.nside);
+    return f;
 }
 
 hipError_t launch_libm(int op, long n, const double *x, const double *y, double *o,

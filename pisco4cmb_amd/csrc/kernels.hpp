@@ -24,6 +24,7 @@ struct PairParams {
     Hpx hb;
     int npix_beam;
     double rho_max, cos_rmax;
+    FastConsts fc;
     const float4 *beam;
     /* pointing (float radians; pa counter-clockwise as Scan holds it) */
     const float *ra, *dec, *pa;
@@ -54,10 +55,11 @@ hipError_t launch_interpol(const Hpx &h, long n, const double *th, const double 
 hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
                        double radius, double cosr, const long *offsets, int *counts,
                        int *pixels, hipStream_t st);
-hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb, long n,
-                        const float *ra, const float *dec, const float *pa,
-                        const long *offsets, const int *pixels, int *bpix, double *chi,
-                        hipStream_t st);
+hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb,
+                        const FastConsts *fc, long n, const float *ra, const float *dec,
+                        const float *pa, const long *offsets, const int *pixels, int *bpix,
+                        double *chi, hipStream_t st);
+FastConsts fast_consts(const Hpx &hb);
 hipError_t launch_libm(int op, long n, const double *x, const double *y, double *o,
                        hipStream_t st);
 

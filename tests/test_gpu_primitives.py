@@ -127,3 +127,27 @@ def test_get_interpol(gpu, oracle, nside):
     p_h, w_h = oracle.get_interpol(nside, th, ph)
     assert np.array_equal(p_d, p_h)
     assert np.array_equal(w_d.view(np.int64), w_h.view(np.int64))
+
+
+@pytest.mark.parametrize("nside_sky,nside_beam,cap_deg", [(512, 4096, 5.0), (256, 2048, 1.5),
+                                                          (128, 1024, 2.5), (2048, 8192, 0.3),
+                                                          (64, 64, 40.0)])
+def test_match_fast_path_bit_exact(gpu, oracle, nside_sky, nside_beam, cap_deg):
+    """The certified fast form used by the convolution kernels returns exactly
+    the oracle's beam pixels (it falls back to the exact path inside its
+    error margin); cos/sin(2 chi) agree with (float)cos/sin(2 chi) to float
+    rounding."""
+    rng = np.random.default_rng(17)
+    n = 600 if nside_sky <= 512 else 3000
+    ra = rng.uniform(0, 2 * math.pi, n).astype(np.float32)
+    dec = (math.pi / 2 - np.arccos(rng.uniform(-1, 1, n))).astype(np.float32)
+    dec[:3] = [1.5707, -1.5707, 1.5]
+    pa = rng.uniform(-math.pi, math.pi, n).astype(np.float32)
+    off, pix = oracle.query_disc_many(nside_sky, ra, dec, math.radians(cap_deg))
+    b_h, chi_h = oracle.match(nside_sky, nside_beam, ra, dec, pa, off, pix)
+    b_f, c2, s2 = gpu.match_fast(nside_sky, nside_beam, ra, dec, pa, off, pix)
+    assert np.array_equal(b_h, b_f)
+    c2_ref = np.cos(2 * chi_h).astype(np.float32)
+    s2_ref = np.sin(2 * chi_h).astype(np.float32)
+    assert np.max(np.abs(c2 - c2_ref)) <= 2.5e-7
+    assert np.max(np.abs(s2 - s2_ref)) <= 2.5e-7
