@@ -162,8 +162,8 @@ def main():
                    "mean_disc": visits / n},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_pair<%s>" % ("OP_ADJ_MAP" if dominant == "adjoint_map"
-                                               else "OP_FWD"),
+                     "kernel": ("k_adjmap_lds" if dominant == "adjoint_map"
+                                else "k_pair<OP_FWD>"),
                      "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
         "fwd_adj": {"forward_s": t_fwd, "adjoint_s": t_adj,
                     "frac_hbm_fwd_plus_adj": (b_fwd + b_adj) / (t_fwd + t_adj) / HBM_PEAK,

@@ -114,6 +114,14 @@ int pisco_exec_adjoint_map(pisco_ctx *ctx, const float *g_a, const float *g_b,
 int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
                             float *const ga[4], float *const gb[4]);
 
+/* Launch-shape tuning of a context (used by the autotuner; defaults are
+ * sound).  Keys: "adjmap_chunk" (samples per LDS window of the sky adjoint;
+ * 0 selects direct global atomics), "adjmap_cap" (window pixels),
+ * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
+ * per CU); pisco_get_tuning also reads "cu_count". */
+int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
+int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);
+
 /* ---- device-resident entry points (device pointers, asynchronous) ------- */
 
 /* Packs eight float beam maps (a0..a3, b0..b3; a3/b3 may be NULL) into the

@@ -50,6 +50,8 @@ SIGNATURES = {
     "pisco_exec_convolution": (c_int, [c_void_p, c_char, P, P]),
     "pisco_exec_adjoint_map": (c_int, [c_void_p, P, P, P, P, P]),
     "pisco_exec_adjoint_beam": (c_int, [c_void_p, P, P, P, P]),
+    "pisco_set_tuning": (c_int, [c_void_p, ctypes.c_char_p, c_long]),
+    "pisco_get_tuning": (c_int, [c_void_p, ctypes.c_char_p, ctypes.POINTER(c_long)]),
     "pisco_dev_pack_beam": (c_int, [c_void_p, c_void_p, c_long, P, P]),
     "pisco_dev_forward": (c_int, [c_void_p, c_void_p, c_int, P, P, P, c_int, c_long, P,
                                   c_long, P, P, P, P, P]),

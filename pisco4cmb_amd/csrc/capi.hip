@@ -102,6 +102,12 @@ struct pisco_ctx {
     DevBuf<double> acc;           /* FP64 accumulators */
     DevBuf<float> fout;           /* float staging of adjoint outputs */
     std::map<int, double2 *> ring_tabs;
+    int ncu = 256;                /* compute units of the device */
+    /* tuning (pisco_set_tuning) */
+    int adjmap_chunk = 64;        /* samples per LDS window; 0 = direct atomics */
+    int adjmap_cap = 10240;       /* LDS window pixels */
+    int adjmap_rw = 1024;         /* LDS window rings */
+    int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
 };
 
 namespace {
@@ -245,6 +251,52 @@ int pisco_pix2ang(int nside, long pix, double *theta, double *phi)
     return PISCO_OK;
 }
 
+int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
+{
+    if (!c || !key)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    std::string k(key);
+    if (k == "adjmap_chunk") {
+        if (value < 0 || value > 4096)
+            return fail(PISCO_ERR_INVALID, "adjmap_chunk must be in [0, 4096]");
+        c->adjmap_chunk = (int)value;
+    } else if (k == "adjmap_cap" || k == "adjmap_rw") {
+        int cap = k == "adjmap_cap" ? (int)value : c->adjmap_cap;
+        int rw = k == "adjmap_rw" ? (int)value : c->adjmap_rw;
+        if (value < 1 || adjmap_lds_bytes(cap, rw) > 163840)
+            return fail(PISCO_ERR_INVALID, "LDS window does not fit 160 KiB");
+        c->adjmap_cap = cap;
+        c->adjmap_rw = rw;
+    } else if (k == "adjmap_blocks") {
+        if (value < 0 || value > 65536)
+            return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
+        c->adjmap_blocks = (int)value;
+    } else {
+        return fail(PISCO_ERR_INVALID, "unknown tuning key '%s'", key);
+    }
+    return PISCO_OK;
+}
+
+int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
+{
+    if (!c || !key || !value)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    std::string k(key);
+    if (k == "adjmap_chunk")
+        *value = c->adjmap_chunk;
+    else if (k == "adjmap_cap")
+        *value = c->adjmap_cap;
+    else if (k == "adjmap_rw")
+        *value = c->adjmap_rw;
+    else if (k == "adjmap_blocks")
+        *value = c->adjmap_blocks;
+    else if (k == "cu_count")
+        *value = c->ncu;
+    else
+        return fail(PISCO_ERR_INVALID, "unknown tuning key '%s'", key);
+    return PISCO_OK;
+}
+
 int pisco_beam_gaussian_fields(int nside_beam, long npix, double fwhmx, double fwhmy,
                                double phi0, double *magEco, double *phsEco,
                                double *magEcx, double *phsEcx)
@@ -364,6 +416,9 @@ int pisco_create(const pisco_run_config *cfg, pisco_ctx **out)
     pisco_ctx *ctx = new pisco_ctx();
     ctx->cfg = c;
     ctx->device = c.deviceId;
+    hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, c.deviceId);
+    if (ctx->ncu <= 0)
+        ctx->ncu = 256;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx;
@@ -631,8 +686,14 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
     p.gI = c->acc.p;
     p.gQ = c->acc.p + npix;
     p.gU = c->acc.p + 2 * npix;
-    if (g_a || g_b)
-        CHK(run_pair(c, OP_ADJ_MAP, p, st));
+    if (g_a || g_b) {
+        if (c->adjmap_chunk > 0) {
+            int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+            HIPCHK(launch_adjmap_lds(p, c->adjmap_chunk, c->adjmap_cap, c->adjmap_rw, nb, st));
+        } else {
+            CHK(run_pair(c, OP_ADJ_MAP, p, st));
+        }
+    }
     HIPCHK(launch_f64_to_f32(p.gI, gI, npix, st));
     HIPCHK(launch_f64_to_f32(p.gQ, gQ, npix, st));
     HIPCHK(launch_f64_to_f32(p.gU, gU, npix, st));

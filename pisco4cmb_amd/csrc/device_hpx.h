@@ -334,6 +334,34 @@ __host__ __device__ inline int disc_ring(const Hpx &h, const Disc &d, int r, int
     return 1;
 }
 
+/* The same ring of the disc as one unwrapped index range: returns 0 (empty),
+ * 1 (ring positions ip_lo..ip_hi, which may run below 0 or past nr-1 and
+ * are taken modulo nr) or 2 (whole ring). */
+__host__ __device__ inline int disc_ring_span(const Hpx &h, const Disc &d, int r, int &ip_lo,
+                                              int &ip_hi, int &ipix1, int &nr)
+{
+    bool shifted;
+    ring_info(h, r, ipix1, nr, shifted);
+    if (r < d.irmin || r > d.irmax)
+        return 2;
+    double z = ring2z(h, r);
+    double x = (d.cosr - z * d.z0) * d.xa;
+    double ysq = 1 - z * z - x * x;
+    double dphi = (ysq <= 0) ? 0.0 : pl_atan2(sqrt(ysq), x);
+    if (!(dphi > 0))
+        return 0;
+    double shift = shifted ? 0.5 : 0.;
+    double a = nr * kInvTwoPi * (d.phi - dphi) - shift;
+    double b = nr * kInvTwoPi * (d.phi + dphi) - shift;
+    ip_lo = ((a >= 0) ? (int)a : (int)a - 1) + 1;
+    ip_hi = (b >= 0) ? (int)b : (int)b - 1;
+    if (ip_lo > ip_hi)
+        return 0;
+    if (ip_hi - ip_lo + 1 >= nr)
+        return 2;
+    return 1;
+}
+
 /* ---- the beam-frame mapping of one visit -------------------------------- */
 
 /* Per-sample constants of the rotation (sphtrigo.cpp:29-30 evaluated once

@@ -18,6 +18,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_hpx.h"
 #include "kernels.hpp"
 
@@ -201,6 +203,264 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
             if (p.tod_b)
                 p.tod_b[i] = (float)acc_b;
         }
+    }
+}
+
+/* ---- adjoint with respect to the sky, LDS-privatised ---------------------
+ *
+ * A block (16 waves) owns a chunk of K consecutive samples.  Their discs
+ * overlap almost entirely, so the block first computes, per ring, the pixel
+ * window that covers every disc of the chunk (pass 1: ring spans unwrapped
+ * around the chunk's first centre, atomic min/max in LDS), lays the windows
+ * out contiguously in LDS (prefix sum), accumulates every visit's I,Q,U
+ * gradient into LDS with ds_add_f32 (pass 2: wave per sample, as the
+ * forward), and finally adds the non-zero window entries to the FP64 global
+ * gradient (one global atomic per pixel and chunk instead of one per visit).
+ * A chunk whose windows do not fit falls back to direct global atomics. */
+
+struct AdjLayout {
+    int chunk;       /* samples per chunk */
+    int cap;         /* LDS pixel capacity of the window */
+    int rw;          /* ring capacity of the window tables */
+    long nchunks;
+};
+
+constexpr int kAdjWaves = 16;
+constexpr int kAdjBlock = kAdjWaves * kWaveSize;
+constexpr int kBigI = 0x3fffffff;
+
+__device__ inline void adj_visit_values(double ga, double gb, float4 A, float4 B, float c2,
+                                        float s2, double &vI, double &vQ, double &vU)
+{
+    double cc = c2, ss = s2;
+    vI = ga * A.x + gb * B.x;
+    vQ = ga * (A.y * cc - A.z * ss) + gb * (-B.y * cc + B.z * ss);
+    vU = ga * (A.z * cc + A.y * ss) + gb * (-B.z * cc - B.y * ss);
+}
+
+__global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    RunTable *runs = (RunTable *)smem;
+    int *win_lo = (int *)(runs + kAdjWaves);
+    int *win_hi = win_lo + L.rw;
+    int *win_base = win_hi + L.rw;          /* rw + 1 */
+    int *ctl = win_base + L.rw + 1;         /* 0 R0, 1 R1, 2 W, 3 fallback */
+    float *acc = (float *)(ctl + 4);
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWaveSize);
+    const int lane = tid % kWaveSize;
+    RunTable &t = runs[wave];
+
+    long lb = xcd_remap(blockIdx.x, gridDim.x);
+    for (long ch = lb; ch < L.nchunks; ch += gridDim.x) {
+        long s0 = p.s_begin + ch * L.chunk;
+        long s1 = s0 + L.chunk < p.s_end ? s0 + L.chunk : p.s_end;
+        int ns = (int)(s1 - s0);
+
+        /* pass 0: ring range of the chunk */
+        if (tid == 0) {
+            ctl[0] = kBigI;
+            ctl[1] = -1;
+        }
+        __syncthreads();
+        for (int j = tid; j < ns; j += kAdjBlock) {
+            Centre c = centre_make(p.ra[s0 + j], p.dec[s0 + j], 0.f);
+            Disc d;
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+            if (d.r_first <= d.r_last) {
+                atomicMin(&ctl[0], d.r_first);
+                atomicMax(&ctl[1], d.r_last);
+            }
+        }
+        __syncthreads();
+        const int R0 = ctl[0];
+        const int nring = ctl[1] - R0 + 1;
+        if (nring <= 0) {   /* uniform: every thread read the same ctl */
+            __syncthreads();   /* before thread 0 resets ctl for the next chunk */
+            continue;
+        }
+        bool fallback = nring > L.rw;
+        double phi_ref;
+        {
+            Centre c = centre_make(p.ra[s0], p.dec[s0], 0.f);
+            Disc d;
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+            phi_ref = d.phi;
+        }
+        if (!fallback) {
+            for (int k = tid; k < nring; k += kAdjBlock) {
+                win_lo[k] = kBigI;
+                win_hi[k] = -kBigI;
+            }
+            __syncthreads();
+            /* pass 1: windows */
+            for (int j = wave; j < ns; j += kAdjWaves) {
+                Centre c = centre_make(p.ra[s0 + j], p.dec[s0 + j], 0.f);
+                Disc d;
+                disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+                for (int r = d.r_first + lane; r <= d.r_last; r += kWaveSize) {
+                    int lo, hi, ipix1, nr;
+                    int kind = disc_ring_span(p.hs, d, r, lo, hi, ipix1, nr);
+                    if (kind == 0)
+                        continue;
+                    if (kind == 2) {
+                        lo = -kBigI;
+                        hi = kBigI;
+                    } else {
+                        double ref = nr * kInvTwoPi * phi_ref;
+                        double mid = 0.5 * (lo + hi);
+                        int sh = (int)rint((ref - mid) / nr);
+                        lo += sh * nr;
+                        hi += sh * nr;
+                    }
+                    atomicMin(&win_lo[r - R0], lo);
+                    atomicMax(&win_hi[r - R0], hi);
+                }
+            }
+            __syncthreads();
+            /* finalise: window start (mod nr) and length per ring; prefix sum */
+            if (wave == 0) {
+                int carry = 0;
+                for (int k0 = 0; k0 < nring; k0 += kWaveSize) {
+                    int k = k0 + lane;
+                    int len = 0;
+                    if (k < nring) {
+                        int lo = win_lo[k], hi = win_hi[k];
+                        int sp, nr;
+                        bool shf;
+                        ring_info(p.hs, R0 + k, sp, nr, shf);
+                        if (lo == kBigI) {
+                            len = 0;
+                            lo = 0;
+                        } else if (lo <= -kBigI / 2 || hi - lo + 1 >= nr) {
+                            len = nr;
+                            lo = 0;
+                        } else {
+                            len = hi - lo + 1;
+                            lo %= nr;
+                            if (lo < 0)
+                                lo += nr;
+                        }
+                        win_lo[k] = lo;
+                        win_hi[k] = len;
+                    }
+                    int inc = wave_incl_scan(len, lane);
+                    if (k < nring)
+                        win_base[k] = carry + inc - len;
+                    carry += __shfl(inc, kWaveSize - 1, kWaveSize);
+                }
+                if (lane == 0) {
+                    win_base[nring] = carry;
+                    ctl[2] = carry;
+                }
+            }
+            __syncthreads();
+            const int W = ctl[2];
+            fallback = W > L.cap;
+            if (!fallback)
+                for (int k = tid; k < 3 * W; k += kAdjBlock)
+                    acc[k] = 0.f;
+            __syncthreads();
+        }
+
+        /* pass 2: visits, one wave per sample */
+        const int W = fallback ? 0 : ctl[2];
+        for (int j = wave; j < ns; j += kAdjWaves) {
+            long i = s0 + j;
+            Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+            c.ra = uni(c.ra);
+            c.dec = uni(c.dec);
+            c.psi = uni(c.psi);
+            c.cdc = uni(c.cdc);
+            c.sdc = uni(c.sdc);
+            double s2psi, c2psi;
+            pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+            s2psi = uni(s2psi);
+            c2psi = uni(c2psi);
+            Disc d;
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+            double ga = uni(p.g_a ? (double)p.g_a[i] : 0.0);
+            double gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
+            for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+                int total = build_runs(p.hs, d, r0, lane, t);
+                int s = 0;
+                for (int v = lane; v < total; v += kWaveSize) {
+                    while (v >= t.pref[s + 1])
+                        s++;
+                    int pix = t.pix0[s] + (v - t.pref[s]);
+                    int r = t.ring[s];
+                    int sp, nr;
+                    bool sh;
+                    ring_info(p.hs, r, sp, nr, sh);
+                    double ra_pix = pixel_phi(p.hs, r, pix, sp);
+                    double2 cs = p.ring_cs[r];
+                    float c2, s2;
+                    int bp = beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, cs.x, cs.y,
+                                             c2, s2);
+                    if ((unsigned)bp < (unsigned)p.npix_beam) {
+                        float4 A = p.beam[2 * (long)bp];
+                        float4 B = p.beam[2 * (long)bp + 1];
+                        double vI, vQ, vU;
+                        adj_visit_values(ga, gb, A, B, c2, s2, vI, vQ, vU);
+                        if (fallback) {
+                            unsafeAtomicAdd(&p.gI[pix], vI);
+                            unsafeAtomicAdd(&p.gQ[pix], vQ);
+                            unsafeAtomicAdd(&p.gU[pix], vU);
+                        } else if (vI != 0.0 || vQ != 0.0 || vU != 0.0) {
+                            int k = r - R0;
+                            int off = pix - sp - win_lo[k];
+                            if (off < 0)
+                                off += nr;
+                            if (off < win_hi[k]) {   /* inside the ring's window */
+                                int slot = win_base[k] + off;
+                                atomicAdd(&acc[slot], (float)vI);
+                                atomicAdd(&acc[W + slot], (float)vQ);
+                                atomicAdd(&acc[2 * W + slot], (float)vU);
+                            } else {                 /* cannot happen; stay correct */
+                                unsafeAtomicAdd(&p.gI[pix], vI);
+                                unsafeAtomicAdd(&p.gQ[pix], vQ);
+                                unsafeAtomicAdd(&p.gU[pix], vU);
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+            }
+        }
+        __syncthreads();
+
+        /* flush the window */
+        if (!fallback) {
+            for (int slot = tid; slot < W; slot += kAdjBlock) {
+                float vI = acc[slot], vQ = acc[W + slot], vU = acc[2 * W + slot];
+                if (vI == 0.f && vQ == 0.f && vU == 0.f)
+                    continue;
+                int lo = 0, hi = nring;   /* win_base[lo] <= slot < win_base[hi] */
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (win_base[mid] <= slot)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                int sp, nr;
+                bool shf;
+                ring_info(p.hs, R0 + lo, sp, nr, shf);
+                int ip = win_lo[lo] + (slot - win_base[lo]);
+                if (ip >= nr)
+                    ip -= nr;
+                int pix = sp + ip;
+                if (vI != 0.f)
+                    unsafeAtomicAdd(&p.gI[pix], (double)vI);
+                if (vQ != 0.f)
+                    unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
+                if (vU != 0.f)
+                    unsafeAtomicAdd(&p.gU[pix], (double)vU);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -396,6 +656,38 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
     case OP_ADJ_BEAM: hipLaunchKernelGGL(k_pair<OP_ADJ_BEAM>, dim3(g), dim3(kBlock), 0, st, p); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+size_t adjmap_lds_bytes(int cap, int rw)
+{
+    return sizeof(RunTable) * kAdjWaves + sizeof(int) * (3 * (size_t)rw + 1 + 4) +
+           sizeof(float) * 3 * (size_t)cap;
+}
+
+hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
+                             hipStream_t st)
+{
+    long n = p.s_end - p.s_begin;
+    if (n <= 0)
+        return hipSuccess;
+    AdjLayout L;
+    L.chunk = chunk;
+    L.cap = cap;
+    L.rw = rw;
+    L.nchunks = (n + chunk - 1) / chunk;
+    size_t bytes = adjmap_lds_bytes(cap, rw);
+    static size_t configured = 0;
+    if (bytes > configured) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bytes);
+        if (e != hipSuccess)
+            return e;
+        configured = bytes;
+    }
+    long g = std::min<long>(L.nchunks, nblocks);
+    hipLaunchKernelGGL(k_adjmap_lds, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
     return hipGetLastError();
 }
 

@@ -38,6 +38,11 @@ struct PairParams {
 };
 
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
+/* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
+ * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
+hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
+                             hipStream_t st);
+size_t adjmap_lds_bytes(int cap, int rw);
 hipError_t launch_ring_table(const Hpx &h, double2 *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);

@@ -205,6 +205,20 @@ class GPUConvolver:
         if self.ctx is None:
             raise RuntimeError("configure_execution() must be called first")
 
+    def set_tuning(self, key, value):
+        """Launch-shape tuning of this context (see pisco_set_tuning)."""
+        self._need_ctx()
+        rc = lib.pisco_set_tuning(self.ctx, key.encode(), int(value))
+        if rc == _lib.PISCO_ERR_INVALID:
+            raise ValueError(lib.pisco_last_error().decode())
+        check(rc)
+
+    def get_tuning(self, key):
+        self._need_ctx()
+        v = ctypes.c_long()
+        check(lib.pisco_get_tuning(self.ctx, key.encode(), ctypes.byref(v)))
+        return v.value
+
     def update_sky(self, sky):
         self._need_ctx()
         check(lib.pisco_update_sky(self.ctx, sky.nside, _addr(sky.sI), _addr(sky.sQ),
