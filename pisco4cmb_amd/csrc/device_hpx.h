@@ -458,12 +458,36 @@ __device__ __noinline__ double2 e2i_angles(double yb, double xb, double alpha)
     return make_double2(c2c, s2c);
 }
 
+/* pl_sincos for a wavefront: when no lane needs argument reduction the
+ * reduction step is skipped for the whole wave.  For |x| < 0.78 the
+ * reduction computes n = rint(2x/pi) = 0 and returns y = x, ylo = +0 exactly,
+ * so the results are the same bits either way. */
+__device__ inline void wave_sincos(double x, double *s, double *c)
+{
+    if (__any(!(fabs(x) < 0.78))) {
+        pl_sincos(x, s, c);
+        return;
+    }
+    *s = pl_sin_kernel(x, 0.0);
+    *c = pl_cos_kernel(x, 0.0);
+}
+
+/* 1/n to ~1e-16 relative: hardware reciprocal estimate + two Newton steps
+ * (used only where the value feeds float outputs, never an index). */
+__device__ inline double recip_fast(double n)
+{
+    double r = __builtin_amdgcn_rcp(n);
+    r = fma(r, fma(-n, r, 1.0), r);
+    r = fma(r, fma(-n, r, 1.0), r);
+    return r;
+}
+
 __device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const Centre &c,
                                       double c2psi, double s2psi, double ra_pix,
                                       double cdp, double sdp, float &c2, float &s2)
 {
     double sd, cd;
-    pl_sincos(ra_pix - c.ra, &sd, &cd);
+    wave_sincos(ra_pix - c.ra, &sd, &cd);
     double crho = c.sdc * sdp + c.cdc * cdp * cd;
     bool big;   /* rho > 1e-6 (sphtrigo.cpp:40) */
     if (crho < fc.crho_big)
@@ -487,7 +511,7 @@ __device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const
         double wr = xb * xa - yb * ya, wi = xb * ya + yb * xa;
         double n = wr * wr + wi * wi;
         if (n > 1e-280) {
-            double inv = 1.0 / n;
+            double inv = recip_fast(n);
             er = (wr * wr - wi * wi) * inv;
             ei = 2.0 * wr * wi * inv;
         } else {   /* degenerate geometry: take the angles themselves */
