@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--forms", action="store_true",
                    help="also time the plain drop-in forward (config 2)")
+    p.add_argument("--calib-copy-mib", type=int, default=0,
+                   help="run one device-to-device copy of this size first (a kernel of "
+                        "known traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -63,6 +66,12 @@ def main():
     from pisco4cmb_amd._lib import check, lib
 
     dev = torch.device("cuda:0")
+    if args.calib_copy_mib:
+        src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nside_sky, nside_beam, cap_deg, fwhm = 512, 4096, 5.0, 0.33
     n = args.samples
 
