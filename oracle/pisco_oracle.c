@@ -902,3 +902,24 @@ void or_vec_sin(long n, const double *x, double *o) { for (long i = 0; i < n; i+
 void or_vec_cos(long n, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_COS(x[i]); }
 void or_vec_acos(long n, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_ACOS(x[i]); }
 void or_vec_atan2(long n, const double *y, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_ATAN2(y[i], x[i]); }
+
+/* The kernels form phi = ((iphi - 0.5) * pi/2) / rdiv in the polar caps as
+ * q = x * inv, q + (x - q * rdiv) * inv with inv = 1.0 / rdiv.  Returns the
+ * number of cap pixels of NSIDE <= nmax for which that differs from the
+ * IEEE quotient (every rdiv < nmax and every iphi in [1, 4 rdiv]). */
+long or_check_cap_division(int nmax)
+{
+    long bad = 0;
+    for (int r = 1; r < nmax; r++) {
+        double inv = 1.0 / r;
+        double rd = r;
+        for (int iphi = 1; iphi <= 4 * r; iphi++) {
+            double x = (iphi - 0.5) * O_HALFPI;
+            double q = x * inv;
+            double e = fma(-q, rd, x);
+            if (fma(e, inv, q) != x / r)
+                bad++;
+        }
+    }
+    return bad;
+}

@@ -101,7 +101,7 @@ struct pisco_ctx {
     DevBuf<float> cot;            /* 2 * nsamples cotangents */
     DevBuf<double> acc;           /* FP64 accumulators */
     DevBuf<float> fout;           /* float staging of adjoint outputs */
-    std::map<int, double2 *> ring_tabs;
+    std::map<int, RingRec *> ring_tabs;
     int ncu = 256;                /* compute units of the device */
     /* tuning (pisco_set_tuning) */
     int adjmap_chunk = 64;        /* samples per LDS window; 0 = direct atomics */
@@ -120,15 +120,15 @@ hipStream_t pick(pisco_ctx *c, void *s)
     return (hipStream_t)s;
 }
 
-int ring_table(pisco_ctx *c, int nside, const double2 **out)
+int ring_table(pisco_ctx *c, int nside, const RingRec **out)
 {
     auto it = c->ring_tabs.find(nside);
     if (it != c->ring_tabs.end()) {
         *out = it->second;
         return PISCO_OK;
     }
-    double2 *t = nullptr;
-    HIPCHK(hipMalloc((void **)&t, sizeof(double2) * 4 * (size_t)nside));
+    RingRec *t = nullptr;
+    HIPCHK(hipMalloc((void **)&t, sizeof(RingRec) * 4 * (size_t)nside));
     HIPCHK(launch_ring_table(hpx_make(nside), t, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->ring_tabs[nside] = t;
@@ -162,11 +162,11 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
                     "beam npix %ld must be in [1, 12*nside_beam^2 - 1]", npix_beam);
     if (nsamples < 0)
         return fail(PISCO_ERR_INVALID, "negative nsamples");
-    const double2 *rt = nullptr;
+    const RingRec *rt = nullptr;
     CHK(ring_table(c, nside_sky, &rt));
     memset(&p, 0, sizeof p);
     p.hs = hpx_make(nside_sky);
-    p.ring_cs = rt;
+    p.rings = rt;
     p.I = I;
     p.Q = Q;
     p.U = U;
@@ -468,7 +468,7 @@ int pisco_update_sky(pisco_ctx *c, int nside, const float *I, const float *Q,
     HIPCHK(hipMemcpyAsync(c->I.p, I, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->Q.p, Q, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->U.p, U, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    const double2 *rt;
+    const RingRec *rt;
     CHK(ring_table(c, nside, &rt));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->nside_sky = nside;
@@ -794,7 +794,7 @@ int pisco_dev_match_mode(pisco_ctx *c, void *stream, int mode, int nside_sky, in
     if (n == 0)
         return PISCO_OK;
     HIPCHK(hipSetDevice(c->device));
-    const double2 *rt;
+    const RingRec *rt;
     CHK(ring_table(c, nside_sky, &rt));
     Hpx hb = hpx_make(nside_beam);
     FastConsts fc = fast_consts(hb);

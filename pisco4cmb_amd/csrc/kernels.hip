@@ -113,6 +113,16 @@ __device__ inline int build_runs(const Hpx &hs, const Disc &d, int r0, int lane,
     return total;
 }
 
+/* Beam pixel and float cos/sin(2 chi) of the visit of sky pixel pix (ring r)
+ * from the sample whose beam-centre constants are c. */
+__device__ inline int visit_beam(const PairParams &p, const Centre &c, double c2psi,
+                                 double s2psi, int r, int pix, float &c2, float &s2)
+{
+    RingRec rr = p.rings[r];
+    double ra_pix = ring_phi(p.hs, rr, r, pix);
+    return beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
+}
+
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
 {
@@ -153,13 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
                 s++;
             int pix = t.pix0[s] + (v - t.pref[s]);
             int r = t.ring[s];
-            int sp, nr;
-            bool sh;
-            ring_info(p.hs, r, sp, nr, sh);
-            double ra_pix = pixel_phi(p.hs, r, pix, sp);
-            double2 cs = p.ring_cs[r];
             float c2, s2;
-            int bp = beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, cs.x, cs.y, c2, s2);
+            int bp = visit_beam(p, c, c2psi, s2psi, r, pix, c2, s2);
             if ((unsigned)bp < (unsigned)p.npix_beam) {   /* also rejects bp < 0 */
                 float4 A = p.beam[2 * (long)bp];
                 float4 B = p.beam[2 * (long)bp + 1];
@@ -391,14 +396,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                         s++;
                     int pix = t.pix0[s] + (v - t.pref[s]);
                     int r = t.ring[s];
-                    int sp, nr;
-                    bool sh;
-                    ring_info(p.hs, r, sp, nr, sh);
-                    double ra_pix = pixel_phi(p.hs, r, pix, sp);
-                    double2 cs = p.ring_cs[r];
                     float c2, s2;
-                    int bp = beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, cs.x, cs.y,
-                                             c2, s2);
+                    int bp = visit_beam(p, c, c2psi, s2psi, r, pix, c2, s2);
                     if ((unsigned)bp < (unsigned)p.npix_beam) {
                         float4 A = p.beam[2 * (long)bp];
                         float4 B = p.beam[2 * (long)bp + 1];
@@ -410,6 +409,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                             unsafeAtomicAdd(&p.gU[pix], vU);
                         } else if (vI != 0.0 || vQ != 0.0 || vU != 0.0) {
                             int k = r - R0;
+                            int sp = p.rings[r].startpix;
+                            int nr = 4 * (r < p.hs.nside ? r
+                                          : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r
+                                                                : p.hs.nside));
                             int off = pix - sp - win_lo[k];
                             if (off < 0)
                                 off += nr;
@@ -464,18 +467,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     }
 }
 
-/* cos and sin of the declination of every ring (dec = pi/2 - theta), indexed
- * by ring number 1 .. 4N-1 (entry 0 unused). */
-__global__ void k_ring_table(Hpx h, double2 *out)
+/* Ring records of every ring, indexed by ring number 1 .. 4N-1 (entry 0
+ * unused). */
+__global__ void k_ring_table(Hpx h, RingRec *out)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (r >= 4 * h.nside)
         return;
-    double th = ring_theta(h, r);
-    double dec = kPi / 2.0 - th;
-    double s, c;
-    pl_sincos(dec, &s, &c);
-    out[r] = make_double2(c, s);
+    out[r] = ring_record(h, r);
 }
 
 __global__ void k_fill_f64(double *x, long n)
@@ -572,7 +571,7 @@ __global__ void k_disc(Hpx h, long n, const float *ra, const float *dec, double 
  * (the quantity kernel 1 of the reference computes, cudaConv2.cu:59-128). */
 /* fast != 0: the kernels' certified form (beam_pixel_fast); chi[k] then
  * receives float cos(2chi) and sin(2chi) packed as two floats. */
-__global__ void k_match(Hpx hs, const double2 *ring_cs, Hpx hb, FastConsts fc, int fast,
+__global__ void k_match(Hpx hs, const RingRec *rings, Hpx hb, FastConsts fc, int fast,
                         long n, const float *ra, const float *dec, const float *pa,
                         const long *offsets, const int *pixels, int *bpix, double *chi)
 {
@@ -585,19 +584,20 @@ __global__ void k_match(Hpx hs, const double2 *ring_cs, Hpx hb, FastConsts fc, i
     for (long k = offsets[i] + threadIdx.x; k < offsets[i + 1]; k += blockDim.x) {
         int pix = pixels[k];
         int r = pix2ring(hs, pix);
-        int sp, nr;
-        bool sh;
-        ring_info(hs, r, sp, nr, sh);
-        double ra_pix = pixel_phi(hs, r, pix, sp);
-        double2 cs = ring_cs[r];
+        RingRec rr = rings[r];
         if (fast) {
             float c2, s2;
-            bpix[k] = beam_pixel_fast(hb, fc, c, c2psi, s2psi, ra_pix, cs.x, cs.y, c2, s2);
+            double ra_pix = ring_phi(hs, rr, r, pix);
+            bpix[k] = beam_pixel_fast(hb, fc, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
             float2 *o = (float2 *)&chi[k];
             *o = make_float2(c2, s2);
         } else {
+            int sp, nr;
+            bool sh;
+            ring_info(hs, r, sp, nr, sh);
+            double ra_pix = pixel_phi(hs, r, pix, sp);
             double x;
-            bpix[k] = beam_pixel(hb, c, ra_pix, cs.x, cs.y, x);
+            bpix[k] = beam_pixel(hb, c, ra_pix, rr.cdp, rr.sdp, x);
             chi[k] = x;
         }
     }
@@ -691,7 +691,7 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
     return hipGetLastError();
 }
 
-hipError_t launch_ring_table(const Hpx &h, double2 *out, hipStream_t st)
+hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st)
 {
     long n = 4L * h.nside - 1;
     hipLaunchKernelGGL(k_ring_table, dim3(nblocks(n, 256)), dim3(256), 0, st, h, out);
@@ -762,7 +762,7 @@ hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
     return hipGetLastError();
 }
 
-hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb,
+hipError_t launch_match(const Hpx &hs, const RingRec *rings, const Hpx &hb,
                         const FastConsts *fc, long n, const float *ra, const float *dec,
                         const float *pa, const long *offsets, const int *pixels, int *bpix,
                         double *chi, hipStream_t st)
@@ -770,7 +770,7 @@ hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb,
     if (n <= 0)
         return hipSuccess;
     FastConsts f = fc ? *fc : fast_consts(hb);
-    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, ring_cs, hb, f,
+    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, rings, hb, f,
                        fc ? 1 : 0, n, ra, dec, pa, offsets, pixels, bpix, chi);
     return hipGetLastError();
 }

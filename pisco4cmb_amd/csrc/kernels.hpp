@@ -17,7 +17,7 @@ enum { OP_FWD = 0, OP_ADJ_MAP = 1, OP_ADJ_BEAM = 2 };
 struct PairParams {
     /* sky: RING map of NSIDE hs.nside, SoA float I,Q,U */
     Hpx hs;
-    const double2 *ring_cs;   /* (cos dec, sin dec) per ring, index 1..4N-1 */
+    const RingRec *rings;     /* per-ring constants, index 1..4N-1 */
     const float *I, *Q, *U;
     /* beam: first npix_beam RING pixels of a map of NSIDE hb.nside,
      * AoS [npix_beam][2] float4 = (A0,A1,A2,A3), (B0,B1,B2,B3) */
@@ -43,7 +43,7 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st);
 size_t adjmap_lds_bytes(int cap, int rw);
-hipError_t launch_ring_table(const Hpx &h, double2 *out, hipStream_t st);
+hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
 hipError_t launch_unpack_beam_grad(const double *g, long npix, float *const out[8],
@@ -60,7 +60,7 @@ hipError_t launch_interpol(const Hpx &h, long n, const double *th, const double 
 hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
                        double radius, double cosr, const long *offsets, int *counts,
                        int *pixels, hipStream_t st);
-hipError_t launch_match(const Hpx &hs, const double2 *ring_cs, const Hpx &hb,
+hipError_t launch_match(const Hpx &hs, const RingRec *rings, const Hpx &hb,
                         const FastConsts *fc, long n, const float *ra, const float *dec,
                         const float *pa, const long *offsets, const int *pixels, int *bpix,
                         double *chi, hipStream_t st);

@@ -52,6 +52,7 @@ class Oracle:
             "or_adjoint_map": (None, [c_int, c_int, c_long, P, c_long, P, P, P, P, P, P]),
             "or_adjoint_beam": (None, [c_int, P, P, P, c_int, c_long, P, c_long, P, P, P, P, P, P]),
             "or_disc_sizes": (c_long, [c_int, c_double, c_long, P, P, P]),
+            "or_check_cap_division": (c_long, [c_int]),
             "or_vec_sin": (None, [c_long, P, P]),
             "or_vec_cos": (None, [c_long, P, P]),
             "or_vec_acos": (None, [c_long, P, P]),
