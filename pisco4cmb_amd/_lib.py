@@ -17,7 +17,8 @@ except Exception:  # pragma: no cover - torch is optional for host-only use
     torch = None
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpisco.so")
+# PISCO_LIB_PATH selects another build of the same library (A/B timing).
+LIB_PATH = os.environ.get("PISCO_LIB_PATH") or os.path.join(HERE, "libpisco.so")
 
 PISCO_OK = 0
 PISCO_ERR_INVALID = 1

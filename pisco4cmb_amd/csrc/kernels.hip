@@ -113,14 +113,74 @@ __device__ inline int build_runs(const Hpx &hs, const Disc &d, int r0, int lane,
     return total;
 }
 
-/* Beam pixel and float cos/sin(2 chi) of the visit of sky pixel pix (ring r)
- * from the sample whose beam-centre constants are c. */
+/* Beam pixel and float cos/sin(2 chi) of the visit of sky pixel pix (ring r,
+ * ring record rr) from the sample whose beam-centre constants are c. */
 __device__ inline int visit_beam(const PairParams &p, const Centre &c, double c2psi,
-                                 double s2psi, int r, int pix, float &c2, float &s2)
+                                 double s2psi, const RingRec &rr, int r, int pix, float &c2,
+                                 float &s2)
 {
-    RingRec rr = p.rings[r];
     double ra_pix = ring_phi(p.hs, rr, r, pix);
     return beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
+}
+
+/* The lane-strided walk over the visits of one ring pass, software
+ * pipelined by hand: the ring record of visit k+1 is loaded while visit k is
+ * computed, and the beam/sky loads of visit k are consumed (by `use`) only
+ * after visit k+1's geometry, so both memory latencies overlap FP64 work.
+ * use(pix, r, sp, bp, c2, s2, A, B) is called once per visit whose beam
+ * pixel lies inside the beam (r: ring, sp: its first pixel), in the lane's
+ * visit order. */
+struct Pending {
+    int pix, bp, r, sp;
+    float c2, s2;
+    float4 A, B;
+    bool live;
+};
+
+template <typename Use>
+__device__ inline void walk_visits(const PairParams &p, const RunTable &t, int total, int lane,
+                                   const Centre &c, double c2psi, double s2psi, Use use)
+{
+    int s = 0;
+    int v = lane;
+    if (v >= total)
+        return;
+    while (v >= t.pref[s + 1])
+        s++;
+    RingRec rr = p.rings[t.ring[s]];
+    Pending pend;
+    pend.live = false;
+    for (; v < total; v += kWaveSize) {
+        int pix = t.pix0[s] + (v - t.pref[s]);
+        int r = t.ring[s];
+        RingRec cur = rr;
+        /* prefetch the next visit's ring record */
+        int vn = v + kWaveSize;
+        if (vn < total) {
+            while (vn >= t.pref[s + 1])
+                s++;
+            rr = p.rings[t.ring[s]];
+        }
+        float c2, s2;
+        int bp = visit_beam(p, c, c2psi, s2psi, cur, r, pix, c2, s2);
+        Pending nw;
+        nw.live = (unsigned)bp < (unsigned)p.npix_beam;   /* also rejects bp < 0 */
+        nw.pix = pix;
+        nw.r = r;
+        nw.sp = cur.startpix;
+        nw.bp = bp;
+        nw.c2 = c2;
+        nw.s2 = s2;
+        if (nw.live) {
+            nw.A = p.beam[2 * (long)bp];
+            nw.B = p.beam[2 * (long)bp + 1];
+        }
+        if (pend.live)
+            use(pend.pix, pend.r, pend.sp, pend.bp, pend.c2, pend.s2, pend.A, pend.B);
+        pend = nw;
+    }
+    if (pend.live)
+        use(pend.pix, pend.r, pend.sp, pend.bp, pend.c2, pend.s2, pend.A, pend.B);
 }
 
 template <int OP>
@@ -157,46 +217,36 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
 
     for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
         int total = build_runs(p.hs, d, r0, lane, t);
-        int s = 0;
-        for (int v = lane; v < total; v += kWaveSize) {
-            while (v >= t.pref[s + 1])
-                s++;
-            int pix = t.pix0[s] + (v - t.pref[s]);
-            int r = t.ring[s];
-            float c2, s2;
-            int bp = visit_beam(p, c, c2psi, s2psi, r, pix, c2, s2);
-            if ((unsigned)bp < (unsigned)p.npix_beam) {   /* also rejects bp < 0 */
-                float4 A = p.beam[2 * (long)bp];
-                float4 B = p.beam[2 * (long)bp + 1];
-                if (OP == OP_FWD) {
-                    float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
-                    float ta = sI * (A.x) + sQ * (A.y * c2 - A.z * s2) +
-                               sU * (A.z * c2 + A.y * s2);
-                    float tb = sI * (B.x) + sQ * (-B.y * c2 + B.z * s2) +
-                               sU * (-B.z * c2 - B.y * s2);
-                    acc_a += ta;
-                    acc_b += tb;
-                } else if (OP == OP_ADJ_MAP) {
-                    double cc = c2, ss = s2;
-                    double vI = ga * A.x + gb * B.x;
-                    double vQ = ga * (A.y * cc - A.z * ss) + gb * (-B.y * cc + B.z * ss);
-                    double vU = ga * (A.z * cc + A.y * ss) + gb * (-B.z * cc - B.y * ss);
-                    unsafeAtomicAdd(&p.gI[pix], vI);
-                    unsafeAtomicAdd(&p.gQ[pix], vQ);
-                    unsafeAtomicAdd(&p.gU[pix], vU);
-                } else {
-                    double cc = c2, ss = s2;
-                    double q = p.Q[pix], u = p.U[pix], sI = p.I[pix];
-                    double *g = p.gbeam + 8 * (long)bp;
-                    unsafeAtomicAdd(&g[0], ga * sI);
-                    unsafeAtomicAdd(&g[1], ga * (q * cc + u * ss));
-                    unsafeAtomicAdd(&g[2], ga * (-q * ss + u * cc));
-                    unsafeAtomicAdd(&g[4], gb * sI);
-                    unsafeAtomicAdd(&g[5], gb * (-q * cc - u * ss));
-                    unsafeAtomicAdd(&g[6], gb * (q * ss - u * cc));
-                }
+        walk_visits(p, t, total, lane, c, c2psi, s2psi,
+                    [&](int pix, int, int, int bp, float c2, float s2, float4 A, float4 B) {
+            if (OP == OP_FWD) {
+                float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
+                float ta = sI * (A.x) + sQ * (A.y * c2 - A.z * s2) +
+                           sU * (A.z * c2 + A.y * s2);
+                float tb = sI * (B.x) + sQ * (-B.y * c2 + B.z * s2) +
+                           sU * (-B.z * c2 - B.y * s2);
+                acc_a += ta;
+                acc_b += tb;
+            } else if (OP == OP_ADJ_MAP) {
+                double cc = c2, ss = s2;
+                double vI = ga * A.x + gb * B.x;
+                double vQ = ga * (A.y * cc - A.z * ss) + gb * (-B.y * cc + B.z * ss);
+                double vU = ga * (A.z * cc + A.y * ss) + gb * (-B.z * cc - B.y * ss);
+                unsafeAtomicAdd(&p.gI[pix], vI);
+                unsafeAtomicAdd(&p.gQ[pix], vQ);
+                unsafeAtomicAdd(&p.gU[pix], vU);
+            } else {
+                double cc = c2, ss = s2;
+                double q = p.Q[pix], u = p.U[pix], sI = p.I[pix];
+                double *g = p.gbeam + 8 * (long)bp;
+                unsafeAtomicAdd(&g[0], ga * sI);
+                unsafeAtomicAdd(&g[1], ga * (q * cc + u * ss));
+                unsafeAtomicAdd(&g[2], ga * (-q * ss + u * cc));
+                unsafeAtomicAdd(&g[4], gb * sI);
+                unsafeAtomicAdd(&g[5], gb * (-q * cc - u * ss));
+                unsafeAtomicAdd(&g[6], gb * (q * ss - u * cc));
             }
-        }
+        });
         wave_sync();
     }
     if (OP == OP_FWD) {
@@ -390,17 +440,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             double gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
                 int total = build_runs(p.hs, d, r0, lane, t);
-                int s = 0;
-                for (int v = lane; v < total; v += kWaveSize) {
-                    while (v >= t.pref[s + 1])
-                        s++;
-                    int pix = t.pix0[s] + (v - t.pref[s]);
-                    int r = t.ring[s];
-                    float c2, s2;
-                    int bp = visit_beam(p, c, c2psi, s2psi, r, pix, c2, s2);
-                    if ((unsigned)bp < (unsigned)p.npix_beam) {
-                        float4 A = p.beam[2 * (long)bp];
-                        float4 B = p.beam[2 * (long)bp + 1];
+                walk_visits(p, t, total, lane, c, c2psi, s2psi,
+                            [&](int pix, int r, int sp, int, float c2, float s2, float4 A,
+                                float4 B) {
+                    {
                         double vI, vQ, vU;
                         adj_visit_values(ga, gb, A, B, c2, s2, vI, vQ, vU);
                         if (fallback) {
@@ -409,7 +452,6 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                             unsafeAtomicAdd(&p.gU[pix], vU);
                         } else if (vI != 0.0 || vQ != 0.0 || vU != 0.0) {
                             int k = r - R0;
-                            int sp = p.rings[r].startpix;
                             int nr = 4 * (r < p.hs.nside ? r
                                           : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r
                                                                 : p.hs.nside));
@@ -428,7 +470,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                             }
                         }
                     }
-                }
+                });
                 wave_sync();
             }
         }
