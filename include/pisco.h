@@ -125,7 +125,8 @@ int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);
 /* ---- device-resident entry points (device pointers, asynchronous) ------- */
 
 /* Packs eight float beam maps (a0..a3, b0..b3; a3/b3 may be NULL) into the
- * kernels' [npix][8] layout beam8 (8*npix floats, device). */
+ * kernels' row layout beam8: npix rows of A0 A1 A2 B0 B1 B2 A3 B3
+ * (8*npix floats, device). */
 int pisco_dev_pack_beam(pisco_ctx *ctx, void *stream, long npix,
                         const float *const maps[8], float *beam8);
 
@@ -143,8 +144,8 @@ int pisco_dev_adjoint_map(pisco_ctx *ctx, void *stream, int nside_sky,
                           const float *pa, const float *g_a, const float *g_b,
                           float *gI, float *gQ, float *gU);
 
-/* Adjoint w.r.t. the beams: gbeam8 (device, [npix_beam][8] floats, same
- * layout as beam8). */
+/* Adjoint w.r.t. the beams: gbeam8 (device, npix_beam rows of 8 floats in
+ * the beam8 layout; the A3/B3 entries are zero). */
 int pisco_dev_adjoint_beam(pisco_ctx *ctx, void *stream, int nside_sky,
                            const float *I, const float *Q, const float *U,
                            int nside_beam, long npix_beam, long nsamples,

@@ -94,7 +94,7 @@ struct pisco_ctx {
     int nside_beam = 0;
     long npix_beam = 0;
     double rho_max = 0.0;
-    DevBuf<float4> beam;          /* [npix][2] */
+    DevBuf<float> beam;           /* [npix][8]: A0 A1 A2 B0 B1 B2 A3 B3 */
     long nsamples = 0;
     DevBuf<float> ra, dec, pa;
     DevBuf<float> tod;            /* 2 * nsamples */
@@ -175,7 +175,7 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
     p.rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
     p.cos_rmax = cos_radius(p.rho_max);
     p.fc = fast_consts(p.hb);
-    p.beam = (const float4 *)beam8;
+    p.beam = beam8;
     p.ra = ra;
     p.dec = dec;
     p.pa = pa;
@@ -489,15 +489,16 @@ int pisco_update_beam(pisco_ctx *c, int nside_beam, long npix, const float *cons
     if (npix < 1 || npix >= 12L * nside_beam * nside_beam)
         return fail(PISCO_ERR_INVALID, "beam npix %ld out of range", npix);
     HIPCHK(hipSetDevice(c->device));
-    CHK(c->beam.ensure(2 * (size_t)npix));
-    /* host transpose into the [npix][8] layout (as update_beam does into
-     * float4, gpuconvolver.cu:302-340) */
+    CHK(c->beam.ensure(8 * (size_t)npix));
+    /* host transpose into rows A0 A1 A2 B0 B1 B2 A3 B3 (update_beam packs
+     * float4 rows per detector, gpuconvolver.cu:302-340) */
     float *h = nullptr;
     HIPCHK(hipHostMalloc((void **)&h, sizeof(float) * 8 * (size_t)npix, 0));
     for (long i = 0; i < npix; i++) {
         float *o = h + 8 * i;
-        o[0] = a[0][i]; o[1] = a[1][i]; o[2] = a[2][i]; o[3] = a[3] ? a[3][i] : 0.f;
-        o[4] = b[0][i]; o[5] = b[1][i]; o[6] = b[2][i]; o[7] = b[3] ? b[3][i] : 0.f;
+        o[0] = a[0][i]; o[1] = a[1][i]; o[2] = a[2][i];
+        o[3] = b[0][i]; o[4] = b[1][i]; o[5] = b[2][i];
+        o[6] = a[3] ? a[3][i] : 0.f; o[7] = b[3] ? b[3][i] : 0.f;
     }
     hipError_t e = hipMemcpyAsync(c->beam.p, h, sizeof(float) * 8 * (size_t)npix,
                                   hipMemcpyHostToDevice, c->stream);
@@ -621,10 +622,11 @@ int pisco_exec_adjoint_beam(pisco_ctx *c, const float *g_a, const float *g_b,
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) {
+        static const int ia[4] = {0, 1, 2, 6}, ib[4] = {3, 4, 5, 7};
         for (long i = 0; i < nb; i++)
             for (int k = 0; k < 4; k++) {
-                if (ga[k]) ga[k][i] = h[8 * i + k];
-                if (gb[k]) gb[k][i] = h[8 * i + 4 + k];
+                if (ga[k]) ga[k][i] = h[8 * i + ia[k]];
+                if (gb[k]) gb[k][i] = h[8 * i + ib[k]];
             }
     }
     hipHostFree(h);
@@ -645,7 +647,7 @@ int pisco_dev_pack_beam(pisco_ctx *c, void *stream, long npix, const float *cons
     if (npix < 1)
         return fail(PISCO_ERR_INVALID, "npix must be positive");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(launch_pack_beam(npix, maps, (float4 *)beam8, pick(c, stream)));
+    HIPCHK(launch_pack_beam(npix, maps, beam8, pick(c, stream)));
     return PISCO_OK;
 }
 

@@ -476,7 +476,7 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
 struct FastConsts {
     double crho_big;    /* crho <  crho_big  => rho > 1e-6 for sure */
     double crho_small;  /* crho >= crho_small => rho <= 1e-6 for sure */
-    float nside_marg;   /* 100 * 4.4e-16 * nside */
+    double marg_k;      /* 100 * 4.4e-16 * nside * sqrt(3) */
 };
 
 /* Rare paths of beam_pixel_fast, kept out of line so that they do not
@@ -514,14 +514,12 @@ __device__ inline void wave_sincos(double x, double *s, double *c)
     *c = pl_cos_kernel(x, 0.0);
 }
 
-/* 1/n to ~1e-16 relative: hardware reciprocal estimate + two Newton steps
+/* 1/n to ~1e-15 relative: hardware reciprocal estimate + one Newton step
  * (used only where the value feeds float outputs, never an index). */
 __device__ inline double recip_fast(double n)
 {
     double r = __builtin_amdgcn_rcp(n);
-    r = fma(r, fma(-n, r, 1.0), r);
-    r = fma(r, fma(-n, r, 1.0), r);
-    return r;
+    return fma(r, fma(-n, r, 1.0), r);
 }
 
 __device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const Centre &c,
@@ -581,9 +579,9 @@ __device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const
     double u2 = (1.0 - tp) * tmp;
     int jp = (int)u1, jm = (int)u2;
     double d1 = u1 - jp, d2 = u2 - jm;
-    /* 1/sqrt(omc) = sqrt(3) * y */
-    float m = fc.nside_marg * (float)(1.7320508075688772 * y) + 3e-13f * (float)tmp + 1e-10f;
-    double md = (double)m;
+    /* margin: nside * 4.4e-14 / sqrt(omc) + 3e-13 tmp + 1e-10, with
+     * 1/sqrt(omc) = sqrt(3) * y */
+    double md = fma(fc.marg_k, y, fma(3e-13, tmp, 1e-10));
     bool ok = (crho > kTwoThird + 1e-12) && (d1 > md) && (d1 < 1.0 - md) && (d2 > md) &&
               (d2 < 1.0 - md);
     if (ok) {

@@ -127,17 +127,19 @@ __device__ inline int visit_beam(const PairParams &p, const Centre &c, double c2
  * pipelined by hand: the ring record of visit k+1 is loaded while visit k is
  * computed, and the beam/sky loads of visit k are consumed (by `use`) only
  * after visit k+1's geometry, so both memory latencies overlap FP64 work.
- * use(pix, r, sp, bp, c2, s2, A, B) is called once per visit whose beam
- * pixel lies inside the beam (r: ring, sp: its first pixel), in the lane's
- * visit order. */
+ * use(q) is called once per visit whose beam pixel lies inside the beam, in
+ * the lane's visit order, with the visit's Pending record (the sky values
+ * are loaded only when LOAD_SKY). */
 struct Pending {
     int pix, bp, r, sp;
     float c2, s2;
-    float4 A, B;
+    float4 v0;        /* A0 A1 A2 B0 */
+    float2 v1;        /* B1 B2 */
+    float sI, sQ, sU; /* sky of the visit (forward only) */
     bool live;
 };
 
-template <typename Use>
+template <bool LOAD_SKY, typename Use>
 __device__ inline void walk_visits(const PairParams &p, const RunTable &t, int total, int lane,
                                    const Centre &c, double c2psi, double s2psi, Use use)
 {
@@ -172,15 +174,21 @@ __device__ inline void walk_visits(const PairParams &p, const RunTable &t, int t
         nw.c2 = c2;
         nw.s2 = s2;
         if (nw.live) {
-            nw.A = p.beam[2 * (long)bp];
-            nw.B = p.beam[2 * (long)bp + 1];
+            const float *row = p.beam + 8 * (long)bp;
+            nw.v0 = *(const float4 *)row;
+            nw.v1 = *(const float2 *)(row + 4);
+            if (LOAD_SKY) {
+                nw.sI = p.I[pix];
+                nw.sQ = p.Q[pix];
+                nw.sU = p.U[pix];
+            }
         }
         if (pend.live)
-            use(pend.pix, pend.r, pend.sp, pend.bp, pend.c2, pend.s2, pend.A, pend.B);
+            use(pend);
         pend = nw;
     }
     if (pend.live)
-        use(pend.pix, pend.r, pend.sp, pend.bp, pend.c2, pend.s2, pend.A, pend.B);
+        use(pend);
 }
 
 template <int OP>
@@ -217,34 +225,35 @@ __global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
 
     for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
         int total = build_runs(p.hs, d, r0, lane, t);
-        walk_visits(p, t, total, lane, c, c2psi, s2psi,
-                    [&](int pix, int, int, int bp, float c2, float s2, float4 A, float4 B) {
+        walk_visits<OP == OP_FWD>(p, t, total, lane, c, c2psi, s2psi, [&](const Pending &q) {
+            const float A0 = q.v0.x, A1 = q.v0.y, A2 = q.v0.z;
+            const float B0 = q.v0.w, B1 = q.v1.x, B2 = q.v1.y;
+            const float c2 = q.c2, s2 = q.s2;
+            const int pix = q.pix, bp = q.bp;
             if (OP == OP_FWD) {
-                float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
-                float ta = sI * (A.x) + sQ * (A.y * c2 - A.z * s2) +
-                           sU * (A.z * c2 + A.y * s2);
-                float tb = sI * (B.x) + sQ * (-B.y * c2 + B.z * s2) +
-                           sU * (-B.z * c2 - B.y * s2);
+                float sI = q.sI, sQ = q.sQ, sU = q.sU;
+                float ta = sI * (A0) + sQ * (A1 * c2 - A2 * s2) + sU * (A2 * c2 + A1 * s2);
+                float tb = sI * (B0) + sQ * (-B1 * c2 + B2 * s2) + sU * (-B2 * c2 - B1 * s2);
                 acc_a += ta;
                 acc_b += tb;
             } else if (OP == OP_ADJ_MAP) {
                 double cc = c2, ss = s2;
-                double vI = ga * A.x + gb * B.x;
-                double vQ = ga * (A.y * cc - A.z * ss) + gb * (-B.y * cc + B.z * ss);
-                double vU = ga * (A.z * cc + A.y * ss) + gb * (-B.z * cc - B.y * ss);
+                double vI = ga * A0 + gb * B0;
+                double vQ = ga * (A1 * cc - A2 * ss) + gb * (-B1 * cc + B2 * ss);
+                double vU = ga * (A2 * cc + A1 * ss) + gb * (-B2 * cc - B1 * ss);
                 unsafeAtomicAdd(&p.gI[pix], vI);
                 unsafeAtomicAdd(&p.gQ[pix], vQ);
                 unsafeAtomicAdd(&p.gU[pix], vU);
             } else {
                 double cc = c2, ss = s2;
-                double q = p.Q[pix], u = p.U[pix], sI = p.I[pix];
+                double qq = p.Q[pix], u = p.U[pix], sI = p.I[pix];
                 double *g = p.gbeam + 8 * (long)bp;
                 unsafeAtomicAdd(&g[0], ga * sI);
-                unsafeAtomicAdd(&g[1], ga * (q * cc + u * ss));
-                unsafeAtomicAdd(&g[2], ga * (-q * ss + u * cc));
-                unsafeAtomicAdd(&g[4], gb * sI);
-                unsafeAtomicAdd(&g[5], gb * (-q * cc - u * ss));
-                unsafeAtomicAdd(&g[6], gb * (q * ss - u * cc));
+                unsafeAtomicAdd(&g[1], ga * (qq * cc + u * ss));
+                unsafeAtomicAdd(&g[2], ga * (-qq * ss + u * cc));
+                unsafeAtomicAdd(&g[3], gb * sI);
+                unsafeAtomicAdd(&g[4], gb * (-qq * cc - u * ss));
+                unsafeAtomicAdd(&g[5], gb * (qq * ss - u * cc));
             }
         });
         wave_sync();
@@ -284,13 +293,14 @@ constexpr int kAdjWaves = 16;
 constexpr int kAdjBlock = kAdjWaves * kWaveSize;
 constexpr int kBigI = 0x3fffffff;
 
-__device__ inline void adj_visit_values(double ga, double gb, float4 A, float4 B, float c2,
+__device__ inline void adj_visit_values(double ga, double gb, float4 v0, float2 v1, float c2,
                                         float s2, double &vI, double &vQ, double &vU)
 {
     double cc = c2, ss = s2;
-    vI = ga * A.x + gb * B.x;
-    vQ = ga * (A.y * cc - A.z * ss) + gb * (-B.y * cc + B.z * ss);
-    vU = ga * (A.z * cc + A.y * ss) + gb * (-B.z * cc - B.y * ss);
+    float A0 = v0.x, A1 = v0.y, A2 = v0.z, B0 = v0.w, B1 = v1.x, B2 = v1.y;
+    vI = ga * A0 + gb * B0;
+    vQ = ga * (A1 * cc - A2 * ss) + gb * (-B1 * cc + B2 * ss);
+    vU = ga * (A2 * cc + A1 * ss) + gb * (-B2 * cc - B1 * ss);
 }
 
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
@@ -440,12 +450,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             double gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
                 int total = build_runs(p.hs, d, r0, lane, t);
-                walk_visits(p, t, total, lane, c, c2psi, s2psi,
-                            [&](int pix, int r, int sp, int, float c2, float s2, float4 A,
-                                float4 B) {
+                walk_visits<false>(p, t, total, lane, c, c2psi, s2psi, [&](const Pending &q) {
                     {
+                        const int pix = q.pix, r = q.r, sp = q.sp;
                         double vI, vQ, vU;
-                        adj_visit_values(ga, gb, A, B, c2, s2, vI, vQ, vU);
+                        adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
                         if (fallback) {
                             unsafeAtomicAdd(&p.gI[pix], vI);
                             unsafeAtomicAdd(&p.gQ[pix], vQ);
@@ -533,28 +542,15 @@ __global__ void k_f64_to_f32(const double *x, float *y, long n)
         y[i] = (float)x[i];
 }
 
-/* [npix][8] doubles (A0..A3, B0..B3) -> eight float maps */
-__global__ void k_unpack_beam_grad(const double *g, long npix, float *a0, float *a1,
-                                   float *a2, float *a3, float *b0, float *b1,
-                                   float *b2, float *b3)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < npix; i += (long)gridDim.x * blockDim.x) {
-        const double *s = g + 8 * i;
-        a0[i] = (float)s[0]; a1[i] = (float)s[1]; a2[i] = (float)s[2]; a3[i] = (float)s[3];
-        b0[i] = (float)s[4]; b1[i] = (float)s[5]; b2[i] = (float)s[6]; b3[i] = (float)s[7];
-    }
-}
-
-/* eight float maps -> [npix][8] (float4 A, float4 B) */
+/* eight float maps -> rows A0 A1 A2 B0 B1 B2 A3 B3 */
 __global__ void k_pack_beam(long npix, const float *a0, const float *a1, const float *a2,
                             const float *a3, const float *b0, const float *b1,
                             const float *b2, const float *b3, float4 *out)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < npix; i += (long)gridDim.x * blockDim.x) {
-        out[2 * i] = make_float4(a0[i], a1[i], a2[i], a3 ? a3[i] : 0.f);
-        out[2 * i + 1] = make_float4(b0[i], b1[i], b2[i], b3 ? b3[i] : 0.f);
+        out[2 * i] = make_float4(a0[i], a1[i], a2[i], b0[i]);
+        out[2 * i + 1] = make_float4(b1[i], b2[i], a3 ? a3[i] : 0.f, b3 ? b3[i] : 0.f);
     }
 }
 
@@ -756,20 +752,10 @@ hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_unpack_beam_grad(const double *g, long npix, float *const out[8],
-                                   hipStream_t st)
-{
-    hipLaunchKernelGGL(k_unpack_beam_grad, dim3(gs_blocks(npix)), dim3(256), 0, st, g,
-                       npix, out[0], out[1], out[2], out[3], out[4], out[5], out[6],
-                       out[7]);
-    return hipGetLastError();
-}
-
-hipError_t launch_pack_beam(long npix, const float *const in[8], float4 *out,
-                            hipStream_t st)
+hipError_t launch_pack_beam(long npix, const float *const in[8], float *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pack_beam, dim3(gs_blocks(npix)), dim3(256), 0, st, npix, in[0],
-                       in[1], in[2], in[3], in[4], in[5], in[6], in[7], out);
+                       in[1], in[2], in[3], in[4], in[5], in[6], in[7], (float4 *)out);
     return hipGetLastError();
 }
 
@@ -824,7 +810,7 @@ FastConsts fast_consts(const Hpx &hb)
     FastConsts f;
     f.crho_big = pl_cos(1.5e-6);
     f.crho_small = pl_cos(0.5e-6);
-    f.nside_marg = (float)(100.0 * 4.4e-16 * hb.nside);
+    f.marg_k = 100.0 * 4.4e-16 * hb.nside * 1.7320508075688772;
     return f;
 }
 

@@ -19,13 +19,14 @@ struct PairParams {
     Hpx hs;
     const RingRec *rings;     /* per-ring constants, index 1..4N-1 */
     const float *I, *Q, *U;
-    /* beam: first npix_beam RING pixels of a map of NSIDE hb.nside,
-     * AoS [npix_beam][2] float4 = (A0,A1,A2,A3), (B0,B1,B2,B3) */
+    /* beam: first npix_beam RING pixels of a map of NSIDE hb.nside, rows of
+     * eight floats A0 A1 A2 B0 B1 B2 A3 B3 (what the CPU forward reads comes
+     * first: one 16-byte and one 8-byte load per visit) */
     Hpx hb;
     int npix_beam;
     double rho_max, cos_rmax;
     FastConsts fc;
-    const float4 *beam;
+    const float *beam;
     /* pointing (float radians; pa counter-clockwise as Scan holds it) */
     const float *ra, *dec, *pa;
     long s_begin, s_end;
@@ -34,7 +35,7 @@ struct PairParams {
     /* adjoint inputs (TOD cotangents) and outputs (FP64 accumulators) */
     const float *g_a, *g_b;
     double *gI, *gQ, *gU;     /* [npix sky] each */
-    double *gbeam;            /* [npix_beam][8] */
+    double *gbeam;            /* [npix_beam][8], same row layout as beam */
 };
 
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
@@ -46,10 +47,7 @@ size_t adjmap_lds_bytes(int cap, int rw);
 hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
-hipError_t launch_unpack_beam_grad(const double *g, long npix, float *const out[8],
-                                   hipStream_t st);
-hipError_t launch_pack_beam(long npix, const float *const in[8], float4 *out,
-                            hipStream_t st);
+hipError_t launch_pack_beam(long npix, const float *const in[8], float *out, hipStream_t st);
 
 hipError_t launch_pix2ang(const Hpx &h, long n, const int *pix, double *th, double *ph,
                           hipStream_t st);

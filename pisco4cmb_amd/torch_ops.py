@@ -46,7 +46,7 @@ def _req(t, name, dtype=torch.float32):
 
 
 def pack_beam(beam_a, beam_b):
-    """[4,npix] x 2 -> [npix, 8] device layout used by the kernels."""
+    """[4,npix] x 2 -> [npix, 8] rows A0 A1 A2 B0 B1 B2 A3 B3 (kernel layout)."""
     beam_a = _req(beam_a, "beam_a")
     beam_b = _req(beam_b, "beam_b")
     npix = beam_a.shape[1]
@@ -110,8 +110,9 @@ class BeamConvolution(torch.autograd.Function):
                 U.data_ptr(), ctx.nside_beam, npix_beam, n, ra.data_ptr(), dec.data_ptr(),
                 pa.data_ptr(), ga.data_ptr() if ga is not None else None,
                 gb.data_ptr() if gb is not None else None, g8.data_ptr()))
-            gbeam_a = g8[:, :4].t().contiguous()
-            gbeam_b = g8[:, 4:].t().contiguous()
+            # rows A0 A1 A2 B0 B1 B2 A3 B3
+            gbeam_a = g8[:, [0, 1, 2, 6]].t().contiguous()
+            gbeam_b = g8[:, [3, 4, 5, 7]].t().contiguous()
         return gI, gQ, gU, gbeam_a, gbeam_b, None, None, None, None, None
 
 

@@ -124,15 +124,3 @@ def lib_sync(g):
     torch.cuda.synchronize() waits for every stream of the device, which
     orders them before torch reads the results."""
     torch.cuda.synchronize()
-
-
-def beam8_from_maps(a_maps, b_maps):
-    """Host packing of PolBeam aBeams/bBeams into the [npix][8] layout."""
-    a = [np.asarray(x, np.float32) for x in a_maps]
-    b = [np.asarray(x, np.float32) for x in b_maps]
-    npix = a[0].size
-    out = np.zeros((npix, 8), np.float32)
-    for k in range(4):
-        out[:, k] = a[k] if k < len(a) else 0
-        out[:, 4 + k] = b[k] if k < len(b) else 0
-    return out
