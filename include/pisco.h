@@ -118,7 +118,9 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * sound).  Keys: "adjmap_chunk" (samples per LDS window of the sky adjoint;
  * 0 selects direct global atomics), "adjmap_cap" (window pixels),
  * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
- * per CU); pisco_get_tuning also reads "cu_count". */
+ * per CU), "adjmap_support" (1: windows cover only the discs of the beam's
+ * non-zero rows, 0: the full rho_max discs; results are the same either way);
+ * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);
 

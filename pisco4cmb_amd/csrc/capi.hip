@@ -60,7 +60,7 @@ template <typename T> struct DevBuf {
         if (n <= cap)
             return PISCO_OK;
         if (p)
-            hipFree(p);
+            (void)hipFree(p);
         p = nullptr;
         cap = 0;
         hipError_t e = hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T));
@@ -75,7 +75,7 @@ template <typename T> struct DevBuf {
     void release()
     {
         if (p)
-            hipFree(p);
+            (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -108,6 +108,8 @@ struct pisco_ctx {
     int adjmap_cap = 10240;       /* LDS window pixels */
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
+    int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
+    DevBuf<int> support;          /* last non-zero beam row (device) */
 };
 
 namespace {
@@ -221,7 +223,8 @@ const char *pisco_last_error(void) { return g_err.c_str(); }
 int pisco_version(char *buf, int len)
 {
     int rt = 0;
-    hipRuntimeGetVersion(&rt);
+    if (hipRuntimeGetVersion(&rt) != hipSuccess)
+        rt = 0;
     if (buf && len > 0)
         snprintf(buf, (size_t)len, "pisco4cmb_amd 0.1 gfx950 hip-runtime %d", rt);
     return PISCO_OK;
@@ -267,6 +270,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
             return fail(PISCO_ERR_INVALID, "LDS window does not fit 160 KiB");
         c->adjmap_cap = cap;
         c->adjmap_rw = rw;
+    } else if (k == "adjmap_support") {
+        if (value != 0 && value != 1)
+            return fail(PISCO_ERR_INVALID, "adjmap_support must be 0 or 1");
+        c->adjmap_support = (int)value;
     } else if (k == "adjmap_blocks") {
         if (value < 0 || value > 65536)
             return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
@@ -290,6 +297,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjmap_rw;
     else if (k == "adjmap_blocks")
         *value = c->adjmap_blocks;
+    else if (k == "adjmap_support")
+        *value = c->adjmap_support;
     else if (k == "cu_count")
         *value = c->ncu;
     else
@@ -416,8 +425,9 @@ int pisco_create(const pisco_run_config *cfg, pisco_ctx **out)
     pisco_ctx *ctx = new pisco_ctx();
     ctx->cfg = c;
     ctx->device = c.deviceId;
-    hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, c.deviceId);
-    if (ctx->ncu <= 0)
+    if (hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, c.deviceId) !=
+            hipSuccess ||
+        ctx->ncu <= 0)
         ctx->ncu = 256;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -432,8 +442,9 @@ void pisco_destroy(pisco_ctx *ctx)
 {
     if (!ctx)
         return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    /* teardown: errors are not reportable here */
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     ctx->I.release();
     ctx->Q.release();
     ctx->U.release();
@@ -445,9 +456,10 @@ void pisco_destroy(pisco_ctx *ctx)
     ctx->cot.release();
     ctx->acc.release();
     ctx->fout.release();
+    ctx->support.release();
     for (auto &kv : ctx->ring_tabs)
-        hipFree(kv.second);
-    hipStreamDestroy(ctx->stream);
+        (void)hipFree(kv.second);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -504,7 +516,7 @@ int pisco_update_beam(pisco_ctx *c, int nside_beam, long npix, const float *cons
                                   hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->stream);
-    hipHostFree(h);
+    (void)hipHostFree(h);
     HIPCHK(e);
     c->nside_beam = nside_beam;
     c->npix_beam = npix;
@@ -629,7 +641,7 @@ int pisco_exec_adjoint_beam(pisco_ctx *c, const float *g_a, const float *g_b,
                 if (gb[k]) gb[k][i] = h[8 * i + ib[k]];
             }
     }
-    hipHostFree(h);
+    (void)hipHostFree(h);
     HIPCHK(e);
     return PISCO_OK;
 }
@@ -691,6 +703,11 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
     if (g_a || g_b) {
         if (c->adjmap_chunk > 0) {
             int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+            if (c->adjmap_support) {
+                CHK(c->support.ensure(1));
+                HIPCHK(launch_beam_support(beam8, npix_beam, c->support.p, st));
+                p.beam_last = c->support.p;
+            }
             HIPCHK(launch_adjmap_lds(p, c->adjmap_chunk, c->adjmap_cap, c->adjmap_rw, nb, st));
         } else {
             CHK(run_pair(c, OP_ADJ_MAP, p, st));

@@ -481,10 +481,12 @@ struct FastConsts {
 
 /* Rare paths of beam_pixel_fast, kept out of line so that they do not
  * occupy registers in the hot loop. */
-__device__ __noinline__ int beam_pixel_exact_rho(const Hpx &hb, double crho, double sigma)
+/* (NSIDE by value: a reference argument of an out-of-line call would force
+ * the caller's parameter block into addressable scratch memory.) */
+__device__ __noinline__ int beam_pixel_exact_rho(int nside_beam, double crho, double sigma)
 {
     double rho = (crho >= 1.0) ? 0.0 : pl_acos(crho);
-    return ang2pix(hb, rho, sigma);
+    return ang2pix(hpx_make(nside_beam), rho, sigma);
 }
 
 __device__ __noinline__ bool rho_above_1e6(double crho)
@@ -589,7 +591,7 @@ __device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const
         int ip = (int)(tt * ir);
         return 2 * ir * (ir - 1) + ip;
     }
-    return beam_pixel_exact_rho(hb, crho, sigma);
+    return beam_pixel_exact_rho(hb.nside, crho, sigma);
 }
 
 /* get_interpol (Healpix_Base::get_interpol; the device port in the reference

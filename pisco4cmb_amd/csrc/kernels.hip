@@ -192,7 +192,7 @@ __device__ inline void walk_visits(const PairParams &p, const RunTable &t, int t
 }
 
 template <int OP>
-__global__ __launch_bounds__(kBlock) void k_pair(PairParams p)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pair(PairParams p)
 {
     __shared__ RunTable tables[kWavesPerBlock];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
@@ -289,18 +289,22 @@ struct AdjLayout {
     long nchunks;
 };
 
-constexpr int kAdjWaves = 16;
+#ifndef PISCO_ADJ_WAVES
+#define PISCO_ADJ_WAVES 16
+#endif
+constexpr int kAdjWaves = PISCO_ADJ_WAVES;   /* waves per block of the sky adjoint */
 constexpr int kAdjBlock = kAdjWaves * kWaveSize;
 constexpr int kBigI = 0x3fffffff;
 
-__device__ inline void adj_visit_values(double ga, double gb, float4 v0, float2 v1, float c2,
-                                        float s2, double &vI, double &vQ, double &vU)
+/* I,Q,U gradient of one visit, in float: the values are accumulated in float
+ * LDS windows, so FP64 products would be rounded away there anyway. */
+__device__ inline void adj_visit_values(float ga, float gb, float4 v0, float2 v1, float c2,
+                                        float s2, float &vI, float &vQ, float &vU)
 {
-    double cc = c2, ss = s2;
     float A0 = v0.x, A1 = v0.y, A2 = v0.z, B0 = v0.w, B1 = v1.x, B2 = v1.y;
     vI = ga * A0 + gb * B0;
-    vQ = ga * (A1 * cc - A2 * ss) + gb * (-B1 * cc + B2 * ss);
-    vU = ga * (A2 * cc + A1 * ss) + gb * (-B2 * cc - B1 * ss);
+    vQ = ga * (A1 * c2 - A2 * s2) + gb * (-B1 * c2 + B2 * s2);
+    vU = ga * (A2 * c2 + A1 * s2) + gb * (-B2 * c2 - B1 * s2);
 }
 
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
@@ -318,6 +322,27 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     const int lane = tid % kWaveSize;
     RunTable &t = runs[wave];
 
+    /* Window radius.  A visit adds a non-zero value only if its beam pixel
+     * is at or before row `last`, i.e. lies in a beam ring <= R of that row.
+     * A point in a ring-R pixel has colatitude below the centre colatitude of
+     * ring R+1 (pixel vertices lie on the neighbouring rings), so every
+     * contributing sky pixel is within theta(R+1) of the sample; the windows
+     * use theta(R+2), one more ring of margin.  Visits outside a window still
+     * take the global-atomic path below, so the windows only need to be
+     * right for speed, not for the result. */
+    double rho_w = p.rho_max, cos_w = p.cos_rmax;
+    if (p.beam_last) {
+        int last = __builtin_amdgcn_readfirstlane(*p.beam_last);
+        int rq = (last < 0 ? 0 : pix2ring(p.hb, last)) + 2;
+        if (rq < 4 * p.hb.nside) {
+            double th = ring_theta(p.hb, rq);
+            if (th < rho_w) {
+                rho_w = th;
+                cos_w = pl_cos(th);
+            }
+        }
+    }
+
     long lb = xcd_remap(blockIdx.x, gridDim.x);
     for (long ch = lb; ch < L.nchunks; ch += gridDim.x) {
         long s0 = p.s_begin + ch * L.chunk;
@@ -333,7 +358,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
         for (int j = tid; j < ns; j += kAdjBlock) {
             Centre c = centre_make(p.ra[s0 + j], p.dec[s0 + j], 0.f);
             Disc d;
-            disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, rho_w, cos_w, d);
             if (d.r_first <= d.r_last) {
                 atomicMin(&ctl[0], d.r_first);
                 atomicMax(&ctl[1], d.r_last);
@@ -342,16 +367,15 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
         __syncthreads();
         const int R0 = ctl[0];
         const int nring = ctl[1] - R0 + 1;
-        if (nring <= 0) {   /* uniform: every thread read the same ctl */
-            __syncthreads();   /* before thread 0 resets ctl for the next chunk */
-            continue;
-        }
-        bool fallback = nring > L.rw;
+        /* no window ring (a disc of radius rho_w can miss every ring centre
+         * while the rho_max discs still have visits) or too many: global
+         * atomics for this chunk */
+        bool fallback = nring <= 0 || nring > L.rw;
         double phi_ref;
         {
             Centre c = centre_make(p.ra[s0], p.dec[s0], 0.f);
             Disc d;
-            disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, rho_w, cos_w, d);
             phi_ref = d.phi;
         }
         if (!fallback) {
@@ -364,7 +388,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             for (int j = wave; j < ns; j += kAdjWaves) {
                 Centre c = centre_make(p.ra[s0 + j], p.dec[s0 + j], 0.f);
                 Disc d;
-                disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+                disc_setup(p.hs, kHalfPi - c.dec, c.ra, rho_w, cos_w, d);
                 for (int r = d.r_first + lane; r <= d.r_last; r += kWaveSize) {
                     int lo, hi, ipix1, nr;
                     int kind = disc_ring_span(p.hs, d, r, lo, hi, ipix1, nr);
@@ -446,38 +470,37 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             c2psi = uni(c2psi);
             Disc d;
             disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
-            double ga = uni(p.g_a ? (double)p.g_a[i] : 0.0);
-            double gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
+            float ga = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, p.g_a ? p.g_a[i] : 0.f)));
+            float gb = __builtin_bit_cast(
+                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, p.g_b ? p.g_b[i] : 0.f)));
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
                 int total = build_runs(p.hs, d, r0, lane, t);
                 walk_visits<false>(p, t, total, lane, c, c2psi, s2psi, [&](const Pending &q) {
-                    {
-                        const int pix = q.pix, r = q.r, sp = q.sp;
-                        double vI, vQ, vU;
-                        adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
-                        if (fallback) {
-                            unsafeAtomicAdd(&p.gI[pix], vI);
-                            unsafeAtomicAdd(&p.gQ[pix], vQ);
-                            unsafeAtomicAdd(&p.gU[pix], vU);
-                        } else if (vI != 0.0 || vQ != 0.0 || vU != 0.0) {
-                            int k = r - R0;
-                            int nr = 4 * (r < p.hs.nside ? r
-                                          : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r
-                                                                : p.hs.nside));
-                            int off = pix - sp - win_lo[k];
-                            if (off < 0)
-                                off += nr;
-                            if (off < win_hi[k]) {   /* inside the ring's window */
-                                int slot = win_base[k] + off;
-                                atomicAdd(&acc[slot], (float)vI);
-                                atomicAdd(&acc[W + slot], (float)vQ);
-                                atomicAdd(&acc[2 * W + slot], (float)vU);
-                            } else {                 /* cannot happen; stay correct */
-                                unsafeAtomicAdd(&p.gI[pix], vI);
-                                unsafeAtomicAdd(&p.gQ[pix], vQ);
-                                unsafeAtomicAdd(&p.gU[pix], vU);
-                            }
-                        }
+                    const int pix = q.pix, r = q.r;
+                    float vI, vQ, vU;
+                    adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
+                    if (vI == 0.f && vQ == 0.f && vU == 0.f)
+                        return;   /* adds nothing */
+                    const int k = r - R0;
+                    int slot = -1;
+                    if (!fallback && (unsigned)k < (unsigned)nring) {
+                        int nr = 4 * (r < p.hs.nside ? r
+                                      : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r : p.hs.nside));
+                        int off = pix - q.sp - win_lo[k];
+                        if (off < 0)
+                            off += nr;
+                        if (off < win_hi[k])   /* inside the ring's window */
+                            slot = win_base[k] + off;
+                    }
+                    if (slot >= 0) {
+                        atomicAdd(&acc[slot], vI);
+                        atomicAdd(&acc[W + slot], vQ);
+                        atomicAdd(&acc[2 * W + slot], vU);
+                    } else {   /* no window holds the pixel: global FP64 atomics */
+                        unsafeAtomicAdd(&p.gI[pix], (double)vI);
+                        unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
+                        unsafeAtomicAdd(&p.gU[pix], (double)vU);
                     }
                 });
                 wave_sync();
@@ -516,6 +539,27 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
         }
         __syncthreads();
     }
+}
+
+__global__ void k_fill_i32(int *x, int v) { *x = v; }
+
+/* Last row with a non-zero value in columns 0..5 (wave max, one atomic per
+ * wave). */
+__global__ void k_beam_support(const float4 *beam8, long npix, int *last)
+{
+    int best = -1;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < npix; i += (long)gridDim.x * blockDim.x) {
+        float4 a = beam8[2 * i];
+        float4 b = beam8[2 * i + 1];
+        if (a.x != 0.f || a.y != 0.f || a.z != 0.f || a.w != 0.f || b.x != 0.f || b.y != 0.f)
+            best = (int)i;
+    }
+#pragma unroll
+    for (int m = kWaveSize / 2; m > 0; m >>= 1)
+        best = max(best, __shfl_xor(best, m, kWaveSize));
+    if (threadIdx.x % kWaveSize == 0 && best >= 0)
+        atomicMax(last, best);
 }
 
 /* Ring records of every ring, indexed by ring number 1 .. 4N-1 (entry 0
@@ -726,6 +770,15 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
     }
     long g = std::min<long>(L.nchunks, nblocks);
     hipLaunchKernelGGL(k_adjmap_lds, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, last, -1);
+    if (npix > 0)
+        hipLaunchKernelGGL(k_beam_support, dim3(gs_blocks(npix)), dim3(256), 0, st,
+                           (const float4 *)beam8, npix, last);
     return hipGetLastError();
 }
 

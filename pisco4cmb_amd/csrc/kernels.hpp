@@ -27,6 +27,10 @@ struct PairParams {
     double rho_max, cos_rmax;
     FastConsts fc;
     const float *beam;
+    /* device int: index of the last beam row whose A0 A1 A2 B0 B1 B2 are not
+     * all zero (-1: none), or NULL.  Only sizes the sky adjoint's LDS
+     * windows; every visit of the rho_max disc is still computed. */
+    const int *beam_last;
     /* pointing (float radians; pa counter-clockwise as Scan holds it) */
     const float *ra, *dec, *pa;
     long s_begin, s_end;
@@ -44,6 +48,9 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st);
 size_t adjmap_lds_bytes(int cap, int rw);
+/* *last = index of the last row of beam8 [npix][8] with a non-zero value in
+ * columns 0..5 (-1 if none). */
+hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st);
 hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
