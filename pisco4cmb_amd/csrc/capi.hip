@@ -105,7 +105,7 @@ struct pisco_ctx {
     int ncu = 256;                /* compute units of the device */
     /* tuning (pisco_set_tuning) */
     int adjmap_chunk = 64;        /* samples per LDS window; 0 = direct atomics */
-    int adjmap_cap = 10240;       /* LDS window pixels */
+    int adjmap_cap = 6144;        /* LDS window pixels */
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
@@ -177,6 +177,7 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
     p.rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
     p.cos_rmax = cos_radius(p.rho_max);
     p.fc = fast_consts(p.hb);
+    p.fc64 = fast_consts64(p.hb);
     p.beam = beam8;
     p.ra = ra;
     p.dec = dec;
@@ -708,7 +709,14 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
                 HIPCHK(launch_beam_support(beam8, npix_beam, c->support.p, st));
                 p.beam_last = c->support.p;
             }
-            HIPCHK(launch_adjmap_lds(p, c->adjmap_chunk, c->adjmap_cap, c->adjmap_rw, nb, st));
+            hipError_t e =
+                launch_adjmap_lds(p, c->adjmap_chunk, c->adjmap_cap, c->adjmap_rw, nb, st);
+            if (e != hipSuccess) {
+                char d[256];
+                adjmap_describe(d, sizeof d, c->adjmap_cap, c->adjmap_rw);
+                return fail(PISCO_ERR_DEVICE, "sky adjoint launch failed: %s (%s)",
+                            hipGetErrorString(e), d);
+            }
         } else {
             CHK(run_pair(c, OP_ADJ_MAP, p, st));
         }
@@ -815,9 +823,9 @@ int pisco_dev_match_mode(pisco_ctx *c, void *stream, int mode, int nside_sky, in
     HIPCHK(hipSetDevice(c->device));
     const RingRec *rt;
     CHK(ring_table(c, nside_sky, &rt));
-    Hpx hb = hpx_make(nside_beam);
-    FastConsts fc = fast_consts(hb);
-    HIPCHK(launch_match(hpx_make(nside_sky), rt, hb, mode ? &fc : nullptr, n, ra, dec, pa,
+    if (mode < 0 || mode > 3)
+        return fail(PISCO_ERR_INVALID, "match mode must be 0, 1, 2 or 3");
+    HIPCHK(launch_match(hpx_make(nside_sky), rt, hpx_make(nside_beam), mode, n, ra, dec, pa,
                         offsets, pixels, bpix, chi, pick(c, stream)));
     return PISCO_OK;
 }

@@ -452,7 +452,183 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
     return ang2pix(hb, rho, sigma);
 }
 
-/* ---- certified fast form of the same mapping ----------------------------- */
+/* ---- single-precision form of the same mapping, certified ---------------
+ *
+ * The kernels evaluate the visit geometry in float32 and keep the beam pixel
+ * it gives only where no floor of ang2pix can flip; any other visit is
+ * re-evaluated by the exact FP64 path (beam_pixel).  Either way the pixel is
+ * the one the CPU path computes.
+ *
+ * Geometry in haversine form (no cancellation).  With dD = dec_p - dec_c and
+ * h2 = 1 - cos(dra) = 2 sin^2(dra/2):
+ *   1 - cos rho = (1 - cos dD) + cos dec_c cos dec_p h2,
+ *   1 - cos dD  = sin^2 dD / (1 + cos dD),
+ *   ya = sin dra cos dec_p,  xa =  sin dD + cos dec_p sin dec_c h2,
+ *   yb = sin dra cos dec_c,  xb = -sin dD + cos dec_c sin dec_p h2,
+ * alpha = atan2(ya, xa), beta = atan2(yb, xb) (sphtrigo.cpp:41-42).  sin dD
+ * and cos dD come from FP64 products of the ring's and the sample's sin/cos
+ * dec; everything after that is float.  In the polar branch of ang2pix
+ * (cos rho > 2/3) the floors are jp = (int)(tp*tmp), jm = (int)((1-tp)*tmp)
+ * and ip = (int)(tt*(jp+jm+1)), with tt = (2 - 2 alpha/pi - 2 psi/pi) mod 4,
+ * tp = frac(tt), tmp = nside*sqrt(3 (1 - cos rho)).
+ *
+ * Error of the float values against the CPU path's (float32 emulation over
+ * 4e6 random visits, discs of 5 degrees, a quarter of them near the poles):
+ *   tt   <= 6.0e-7 absolute              bound used: 4e-6
+ *   tmp  <= 2.6e-7 relative              bound used: 2e-6
+ * plus the CPU path's own rounding of acos(crho) (|crho error| <= 7e-16), which
+ * moves its tmp by <= 1.05e-15 nside / sqrt(3 (1 - cos rho)); bound used: 100
+ * times that (marg_k).  A floor is kept only if its argument is farther from
+ * an integer than these bounds.  rho > 1e-6 (sphtrigo.cpp:40) is decided in
+ * float outside 0.5e-6 < rho < 1.5e-6 and by the exact path inside.
+ *
+ * chi is needed as float cos(2 chi), sin(2 chi): with chi = beta + alpha + psi
+ * - pi (mod 2 pi), e^{2 i chi} = w^2/|w|^2 e^{2 i psi}, w = (xb + i yb)(xa +
+ * i ya), evaluated in float (error <= 9.2e-7 measured). */
+struct FastConsts {
+    float omc_lo;   /* 1 - cos(0.5e-6): below, rho <= 1e-6 for sure */
+    float omc_hi;   /* 1 - cos(1.5e-6): above, rho >  1e-6 for sure */
+    float marg_k;   /* 1.2e-13 * nside_beam */
+    float nside;    /* nside_beam */
+};
+
+/* Wave-uniform float constants of one sample. */
+struct SampleF {
+    float cdc, sdc;   /* cos, sin dec of the beam centre */
+    float t0;         /* (pi - psi) * 2/pi mod 4 */
+    float c2p, s2p;   /* cos, sin 2 psi */
+};
+
+__host__ __device__ inline SampleF sample_f32(const Centre &c, double c2psi, double s2psi)
+{
+    SampleF f;
+    f.cdc = (float)c.cdc;
+    f.sdc = (float)c.sdc;
+    f.t0 = (float)fmodulo((kPi - c.psi) * kInvHalfPi, 4.0);
+    f.c2p = (float)c2psi;
+    f.s2p = (float)s2psi;
+    return f;
+}
+
+constexpr int kNeedExact = -2;   /* beam_pixel_f32: the floors are not certain */
+
+struct ExactVisit {
+    int bp;
+    float c2, s2;
+};
+
+/* The exact FP64 mapping (beam_pixel) of one visit, with (float)cos/sin 2chi
+ * as the CPU path forms them.  Out of line, scalar arguments only (a
+ * reference argument would force the caller's parameters into scratch). */
+__device__ __noinline__ ExactVisit beam_visit_exact(int nside_beam, double ra_c, double psi,
+                                                    double cdc, double sdc, double ra_pix,
+                                                    double cdp, double sdp)
+{
+    Centre c;
+    c.ra = ra_c;
+    c.dec = 0.0;   /* not read by beam_pixel */
+    c.psi = psi;
+    c.cdc = cdc;
+    c.sdc = sdc;
+    double chi, s, co;
+    ExactVisit e;
+    e.bp = beam_pixel(hpx_make(nside_beam), c, ra_pix, cdp, sdp, chi);
+    pl_sincos(2.0 * chi, &s, &co);
+    e.c2 = (float)co;
+    e.s2 = (float)s;
+    return e;
+}
+
+/* The part of a visit's float geometry that reads the sky pixel's ring
+ * (FP64 in, float out); the rest (beam_pixel_f32_tail) is float only. */
+struct VisitRing {
+    float xh;         /* dra / 2, dra = ra_pix - ra_centre in [-pi, pi] */
+    float sD, opcD;   /* sin(dec_p - dec_c), 1 + cos(dec_p - dec_c) */
+    float cdp, sdp;   /* cos, sin dec_p */
+};
+
+__device__ inline VisitRing visit_ring_f32(const Centre &c, double ra_pix, double cdp, double sdp)
+{
+    double dra = ra_pix - c.ra;
+    dra = fma(-kTwoPi, rint(dra * kInvTwoPi), dra);
+    VisitRing g;
+    g.xh = (float)(0.5 * dra);
+    g.sD = (float)fma(sdp, c.cdc, -(cdp * c.sdc));
+    g.opcD = 1.0f + (float)fma(cdp, c.cdc, sdp * c.sdc);
+    g.cdp = (float)cdp;
+    g.sdp = (float)sdp;
+    return g;
+}
+
+/* Beam pixel and float cos/sin 2chi of a visit, or kNeedExact (c2, s2 then
+ * unset). */
+__device__ inline int beam_pixel_f32_tail(const FastConsts &fc, const SampleF &sf,
+                                          const VisitRing &g, float &c2, float &s2)
+{
+    float sh, ch;
+    plf_sincos(g.xh, &sh, &ch);
+    const float sdra = 2.0f * (sh * ch);
+    const float h2 = 2.0f * (sh * sh);
+    const float omD = (g.sD * g.sD) * __builtin_amdgcn_rcpf(g.opcD);
+    const float omc = fmaf(h2, sf.cdc * g.cdp, omD);   /* 1 - cos rho */
+    const float ya = sdra * g.cdp, xa = fmaf(h2, g.cdp * sf.sdc, g.sD);
+    const float yb = sdra * sf.cdc, xb = fmaf(h2, sf.cdc * g.sdp, -g.sD);
+
+    const bool big = omc > fc.omc_hi;
+    if (!big && !(omc < fc.omc_lo))
+        return kNeedExact;   /* rho within 0.5e-6 of 1e-6 */
+    float alpha = 0x1.921fb6p+0f;   /* pi/2 when rho <= 1e-6 */
+    float er = 1.0f, ei = 0.0f;
+    if (big) {
+        alpha = plf_atan2_core(ya, xa, __builtin_amdgcn_rcpf(fmaxf(fabsf(xa), fabsf(ya))));
+        const float wr = xb * xa - yb * ya, wi = xb * ya + yb * xa;
+        const float n = wr * wr + wi * wi;
+        if (!(n > 1e-36f))
+            return kNeedExact;   /* |w|^2 >= 5e-24 for rho > 1.5e-6 */
+        const float inv = __builtin_amdgcn_rcpf(n);
+        er = (wr * wr - wi * wi) * inv;
+        ei = 2.0f * (wr * wi) * inv;
+    }
+    c2 = er * sf.c2p - ei * sf.s2p;
+    s2 = er * sf.s2p + ei * sf.c2p;
+
+    float tt = fmaf(-alpha, 0x1.45f306p-1f /* 2/pi */, sf.t0);
+    tt = (tt < 0.0f) ? tt + 4.0f : tt;
+    tt = (tt >= 4.0f) ? tt - 4.0f : tt;
+    const float x3 = 3.0f * omc;
+    const float y = __builtin_amdgcn_rsqf(x3);   /* 1/sqrt(3 (1 - cos rho)) */
+    const float tmp = fc.nside * (x3 * y);
+    const float tp = tt - (float)(int)tt;
+    const float u1 = tp * tmp, u2 = (1.0f - tp) * tmp;
+    const int jp = (int)u1, jm = (int)u2;
+    const float d1 = u1 - (float)jp, d2 = u2 - (float)jm;
+    const float md = fmaf(tmp, 8e-6f, fmaf(fc.marg_k, y, 1e-6f));
+    const int ir = jp + jm + 1;
+    const float v = tt * (float)ir;
+    const int ip = (int)v;
+    const float dv = v - (float)ip;
+    const float mv = fmaf((float)ir, 4.5e-6f, 1e-6f);
+    const bool ok = (omc > 0.0f) && (omc < 0.33332f) && (d1 > md) && (d1 < 1.0f - md) &&
+                    (d2 > md) && (d2 < 1.0f - md) && (dv > mv) && (dv < 1.0f - mv);
+    return ok ? 2 * ir * (ir - 1) + ip : kNeedExact;
+}
+
+/* Beam pixel and float cos/sin 2chi of the visit of the sky pixel at
+ * (ra_pix, cos/sin dec_pix), or kNeedExact (c2, s2 then unset). */
+__device__ inline int beam_pixel_f32(const FastConsts &fc, const Centre &c, const SampleF &sf,
+                                     double ra_pix, double cdp, double sdp, float &c2,
+                                     float &s2)
+{
+    return beam_pixel_f32_tail(fc, sf, visit_ring_f32(c, ra_pix, cdp, sdp), c2, s2);
+}
+
+/* ---- certified double-precision form of the same mapping -----------------
+ *
+ * A second certified form, used by the forward kernel: on the 5-degree,
+ * NSIDE 512 workload the forward measured faster with it than with the
+ * single-precision form (its beam gathers saw a 77% instead of 54% L2 hit
+ * rate).  Its margins are ~1e-12 relative, so it practically never needs the
+ * exact path, and it evaluates that path in place instead of deferring. */
 
 /* The CPU path obtains the beam pixel from rho = acos(crho) through
  * z = cos(rho) (and sin(rho) within 0.01 rad of the pole); in the polar branch
@@ -461,7 +637,7 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
  * nside*sin(rho)/sqrt((1+z)/3)).  Both forms equal nside*sqrt(3*(1-crho))
  * up to the rounding of acos/cos/sin, which is bounded by
  *     |tmp_exact - tmp_fast| <= nside*4.4e-16/sqrt(1-crho) + 2e-15*tmp
- * (pisco_libm's acos, cos and sin are within 2 ulp).  beam_pixel_fast
+ * (pisco_libm's acos, cos and sin are within 2 ulp).  beam_pixel_f64
  * evaluates the cheap form and accepts its floors only if both fractional
  * parts are farther than 100 times that bound (plus 1e-10) from an integer;
  * otherwise it evaluates the exact path.  Either way the returned pixel is
@@ -473,13 +649,13 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
  * (mod 2 pi; sphtrigo.cpp:41-53), e^{2 i chi} = w^2/|w|^2 * e^{2 i psi} with
  * w = (xb + i yb)(xa + i ya): no transcendental is needed (FP64 algebra,
  * within ~1e-15 of cos/sin(2 chi) before rounding to float). */
-struct FastConsts {
+struct FastConsts64 {
     double crho_big;    /* crho <  crho_big  => rho > 1e-6 for sure */
     double crho_small;  /* crho >= crho_small => rho <= 1e-6 for sure */
     double marg_k;      /* 100 * 4.4e-16 * nside * sqrt(3) */
 };
 
-/* Rare paths of beam_pixel_fast, kept out of line so that they do not
+/* Rare paths of beam_pixel_f64, kept out of line so that they do not
  * occupy registers in the hot loop. */
 /* (NSIDE by value: a reference argument of an out-of-line call would force
  * the caller's parameter block into addressable scratch memory.) */
@@ -524,7 +700,7 @@ __device__ inline double recip_fast(double n)
     return fma(r, fma(-n, r, 1.0), r);
 }
 
-__device__ inline int beam_pixel_fast(const Hpx &hb, const FastConsts &fc, const Centre &c,
+__device__ inline int beam_pixel_f64(const Hpx &hb, const FastConsts64 &fc, const Centre &c,
                                       double c2psi, double s2psi, double ra_pix,
                                       double cdp, double sdp, float &c2, float &s2)
 {

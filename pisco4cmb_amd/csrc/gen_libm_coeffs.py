@@ -12,8 +12,15 @@ This script derives every constant those functions use:
     atan(u) = u + u*z*T(z),         z = u^2 in [0, tan(pi/12)^2]
 * multi-word splits of pi/2, pi/6, pi/3, pi, 5pi/6, 2pi/3 and sqrt(3).
 
+and the float32 kernels of the kernels' single-precision visit geometry
+(pixel indices from it are certified by error margins, never trusted as is):
+    sin(x) = x + x*z*SF(z),         z = x^2 in [0, (pi/2)^2]
+    cos(x) = 1 - z/2 + z^2*CF(z),   z = x^2 in [0, (pi/2)^2]
+    atan(t) = t + t*z*TF(z),        z = t^2 in [0, 1]
+
 Run: python gen_libm_coeffs.py  (prints the C block pasted into pisco_libm.h)
 """
+import struct
 from mpmath import mp, mpf, sin, cos, asin, atan, sqrt, tan, pi, chebyfit
 
 mp.dps = 60
@@ -87,6 +94,15 @@ def main():
         lines.append("#define PL_%s_HI %s" % (nm, hexd(h)))
         lines.append("#define PL_%s_LO %s" % (nm, hexd(l)))
     lines.append("#define PL_TAN_PI_12 %s" % hexd(tan(pi / 12)))
+    for name, g, zmax, deg in (("SF", g_sin, (pi / 2) ** 2 * mpf("1.0005"), 3),
+                               ("CF", g_cos, (pi / 2) ** 2 * mpf("1.0005"), 3),
+                               ("TF", g_atan, mpf("1.0001"), 8)):
+        c, err = fit(g, zmax, deg)
+        lines.append("/* %s(z): float32, degree %d, fit error %.3g on [0, %.6g] */"
+                     % (name, deg, float(err), float(zmax)))
+        for i, v in enumerate(c):
+            f32 = struct.unpack("f", struct.pack("f", v))[0]
+            lines.append("#define PL_%s%d %sf" % (name, i, float(f32).hex()))
     print("\n".join(lines))
 
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 
 #include "device_hpx.h"
 #include "kernels.hpp"
@@ -76,11 +77,14 @@ __device__ inline long xcd_remap(long b, long nblk)
     return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-/* Run table of one wave for one pass over (up to) 64 rings. */
+/* Run table of one wave for one pass over (up to) 64 rings, and the wave's
+ * queue of visits deferred to the exact FP64 mapping (sky pixel, ring). */
 struct RunTable {
     int pref[kMaxRuns + 1];  /* visit offset of each run; pref[nrun] = total */
     int pix0[kMaxRuns];
     int ring[kMaxRuns];
+    int qpix[2 * kWaveSize];
+    int qring[2 * kWaveSize];
 };
 
 /* Builds the run table for rings r0 .. r0+63 (clipped to the disc); returns
@@ -114,22 +118,21 @@ __device__ inline int build_runs(const Hpx &hs, const Disc &d, int r0, int lane,
 }
 
 /* Beam pixel and float cos/sin(2 chi) of the visit of sky pixel pix (ring r,
- * ring record rr) from the sample whose beam-centre constants are c. */
-__device__ inline int visit_beam(const PairParams &p, const Centre &c, double c2psi,
-                                 double s2psi, const RingRec &rr, int r, int pix, float &c2,
-                                 float &s2)
+ * ring record rr) from the sample with beam-centre constants c, sf; or
+ * kNeedExact. */
+__device__ inline int visit_beam(const PairParams &p, const Centre &c, const SampleF &sf,
+                                 const RingRec &rr, int r, int pix, float &c2, float &s2)
 {
     double ra_pix = ring_phi(p.hs, rr, r, pix);
-    return beam_pixel_fast(p.hb, p.fc, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
+    return beam_pixel_f32(p.fc, c, sf, ra_pix, rr.cdp, rr.sdp, c2, s2);
 }
 
-/* The lane-strided walk over the visits of one ring pass, software
- * pipelined by hand: the ring record of visit k+1 is loaded while visit k is
- * computed, and the beam/sky loads of visit k are consumed (by `use`) only
- * after visit k+1's geometry, so both memory latencies overlap FP64 work.
- * use(q) is called once per visit whose beam pixel lies inside the beam, in
- * the lane's visit order, with the visit's Pending record (the sky values
- * are loaded only when LOAD_SKY). */
+/* Wave-uniform double constants of one sample (double-precision mapping). */
+struct SampleD {
+    double c2psi, s2psi;   /* cos, sin 2 psi */
+};
+
+/* A visit whose beam pixel lies inside the beam, with what `use` needs. */
 struct Pending {
     int pix, bp, r, sp;
     float c2, s2;
@@ -139,56 +142,197 @@ struct Pending {
     bool live;
 };
 
-template <bool LOAD_SKY, typename Use>
-__device__ inline void walk_visits(const PairParams &p, const RunTable &t, int total, int lane,
-                                   const Centre &c, double c2psi, double s2psi, Use use)
+__device__ inline int lanes_below(unsigned long long m)
 {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+/* Runs the exact FP64 mapping on queue entries base .. base+cnt-1 (cnt <= 64),
+ * one per lane, and hands the visits inside the beam to use().  Called with
+ * every lane of the wave active. */
+template <bool LOAD_SKY, typename Use>
+__device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, int cnt, int lane,
+                                   const Centre &c, Use &use)
+{
+    wave_sync();
+    if (lane < cnt) {
+        const int pix = t.qpix[base + lane], r = t.qring[base + lane];
+        const RingRec rr = p.rings[r];
+        const double ra_pix = ring_phi(p.hs, rr, r, pix);
+        const ExactVisit e =
+            beam_visit_exact(p.hb.nside, c.ra, c.psi, c.cdc, c.sdc, ra_pix, rr.cdp, rr.sdp);
+        if ((unsigned)e.bp < (unsigned)p.npix_beam) {
+            Pending q;
+            q.pix = pix;
+            q.r = r;
+            q.sp = rr.startpix;
+            q.bp = e.bp;
+            q.c2 = e.c2;
+            q.s2 = e.s2;
+            const float *row = p.beam + 8 * (long)e.bp;
+            q.v0 = *(const float4 *)row;
+            q.v1 = *(const float2 *)(row + 4);
+            if (LOAD_SKY) {
+                q.sI = p.I[pix];
+                q.sQ = p.Q[pix];
+                q.sU = p.U[pix];
+            }
+            q.live = true;
+            use(q);
+        }
+    }
+    /* Leave no load of this (rare) path outstanding: otherwise the merge of
+     * the paths at the caller's loop head looks as if those loads might still
+     * be pending, and the compiler would wait for the main path's in-flight
+     * loads there too. */
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+}
+
+/* The walk over the visits of one ring pass: lane-strided, with every lane
+ * running the same number of iterations (the deferred-visit queue is drained
+ * by the whole wave).  Software pipelined by hand: the ring record of the
+ * lane's next visit is loaded while the current one is computed, and the
+ * beam/sky loads of a visit are issued at the end of its iteration and
+ * consumed (by `use`) after the next visit's geometry.  The record is used
+ * before the next loads are issued into it, so no register copy of a pending
+ * load is needed.  In the forward every global load of the main path is
+ * issued unconditionally (lanes without a visit load row 0 / pixel 0 and
+ * discard it): a load skipped on some path makes the in-order load counter
+ * ambiguous at the loop head, and the compiler then waits for all loads
+ * there instead of only for the ring record (the adjoints, with fewer loads
+ * per visit, measured faster with conditional loads).  Visits the float
+ * mapping cannot certify go to the queue (qn entries, wave-uniform); a full
+ * batch of 64 is drained at once.  use(q) is called once per visit whose
+ * beam pixel lies inside the beam. */
+template <bool LOAD_SKY, bool MAP64, typename Use>
+__device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, int lane,
+                                   const Centre &c, const SampleF &sf, const SampleD &sd,
+                                   int &qn, Use &use)
+{
+    if (total <= 0)
+        return;
     int s = 0;
     int v = lane;
-    if (v >= total)
-        return;
-    while (v >= t.pref[s + 1])
-        s++;
+    if (v < total)
+        while (v >= t.pref[s + 1])
+            s++;
     RingRec rr = p.rings[t.ring[s]];
-    Pending pend;
-    pend.live = false;
-    for (; v < total; v += kWaveSize) {
-        int pix = t.pix0[s] + (v - t.pref[s]);
-        int r = t.ring[s];
-        RingRec cur = rr;
-        /* prefetch the next visit's ring record */
-        int vn = v + kWaveSize;
-        if (vn < total) {
-            while (vn >= t.pref[s + 1])
-                s++;
-            rr = p.rings[t.ring[s]];
-        }
+    /* complete this first load here: a load still pending on the loop-entry
+     * edge would make the compiler wait for the in-flight beam/sky loads at
+     * every use of the ring record inside the loop */
+    __builtin_amdgcn_s_waitcnt(0);
+    Pending q;
+    q.live = false;
+    for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
+        const bool act = v < total;
+        /* a lane past the end repeats the first visit of its run, discarded */
+        const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
+        const int r = t.ring[s];
         float c2, s2;
-        int bp = visit_beam(p, c, c2psi, s2psi, cur, r, pix, c2, s2);
-        Pending nw;
-        nw.live = (unsigned)bp < (unsigned)p.npix_beam;   /* also rejects bp < 0 */
-        nw.pix = pix;
-        nw.r = r;
-        nw.sp = cur.startpix;
-        nw.bp = bp;
-        nw.c2 = c2;
-        nw.s2 = s2;
-        if (nw.live) {
-            const float *row = p.beam + 8 * (long)bp;
-            nw.v0 = *(const float4 *)row;
-            nw.v1 = *(const float2 *)(row + 4);
-            if (LOAD_SKY) {
-                nw.sI = p.I[pix];
-                nw.sQ = p.Q[pix];
-                nw.sU = p.U[pix];
+        int bp, startpix;
+        if constexpr (MAP64) {
+            const RingRec cur = rr;
+            const int vn = v + kWaveSize;
+            if (vn < total)
+                while (vn >= t.pref[s + 1])
+                    s++;
+            rr = p.rings[t.ring[s]];   /* prefetch of the next visit's record */
+            startpix = cur.startpix;
+            bp = beam_pixel_f64(p.hb, p.fc64, c, sd.c2psi, sd.s2psi, ring_phi(p.hs, cur, r, pix),
+                                cur.cdp, cur.sdp, c2, s2);
+        } else {
+            /* the part of the geometry that reads the ring record, then the
+             * prefetch of the next visit's record into the same registers */
+            const VisitRing g = visit_ring_f32(c, ring_phi(p.hs, rr, r, pix), rr.cdp, rr.sdp);
+            startpix = rr.startpix;
+            const int vn = v + kWaveSize;
+            if (vn < total)
+                while (vn >= t.pref[s + 1])
+                    s++;
+            rr = p.rings[t.ring[s]];
+            bp = beam_pixel_f32_tail(p.fc, sf, g, c2, s2);
+        }
+        bp = act ? bp : -1;
+        if constexpr (!MAP64) {   /* the double-precision form never defers */
+            const unsigned long long m = __ballot(bp == kNeedExact);
+            if (m) {
+                if (bp == kNeedExact) {
+                    const int pos = qn + lanes_below(m);
+                    t.qpix[pos] = pix;
+                    t.qring[pos] = r;
+                }
+                qn += __popcll(m);
             }
         }
-        if (pend.live)
-            use(pend);
-        pend = nw;
+        if (q.live)
+            use(q);
+        if constexpr (!MAP64) {
+            /* drain here, where the previous visit is consumed and the next
+             * loads are not yet issued: little is live across the exact
+             * path's out-of-line call */
+            if (qn >= kWaveSize) {
+                drain_exact<LOAD_SKY>(p, t, qn - kWaveSize, kWaveSize, lane, c, use);
+                qn -= kWaveSize;
+            }
+        }
+        q.live = (unsigned)bp < (unsigned)p.npix_beam;   /* rejects -1 and kNeedExact */
+        q.pix = pix;
+        q.r = r;
+        q.sp = startpix;
+        q.bp = bp;
+        q.c2 = c2;
+        q.s2 = s2;
+        if constexpr (LOAD_SKY) {
+            /* forward: loads issued unconditionally (row 0 / pixel 0 when
+             * the visit is dead), see above */
+            const float *row = p.beam + 8 * (long)(q.live ? bp : 0);
+            q.v0 = *(const float4 *)row;
+            q.v1 = *(const float2 *)(row + 4);
+            const int sp = q.live ? pix : 0;
+            q.sI = p.I[sp];
+            q.sQ = p.Q[sp];
+            q.sU = p.U[sp];
+        } else if (q.live) {
+            /* adjoints: only live visits load (measured faster there) */
+            const float *row = p.beam + 8 * (long)bp;
+            q.v0 = *(const float4 *)row;
+            q.v1 = *(const float2 *)(row + 4);
+        }
     }
-    if (pend.live)
-        use(pend);
+    if (q.live)
+        use(q);
+}
+
+/* Wave-uniform copy of a float (to scalar registers). */
+__device__ inline float unif(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
+/* Per-sample setup shared by the visit kernels: beam-centre constants in
+ * scalar registers and the sample's disc. */
+__device__ inline void sample_setup(const PairParams &p, long i, Centre &c, SampleF &sf,
+                                    SampleD &sd, Disc &d)
+{
+    c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+    c.ra = uni(c.ra);
+    c.dec = uni(c.dec);
+    c.psi = uni(c.psi);
+    c.cdc = uni(c.cdc);
+    c.sdc = uni(c.sdc);
+    double s2psi, c2psi;
+    pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+    sd.c2psi = uni(c2psi);
+    sd.s2psi = uni(s2psi);
+    sf = sample_f32(c, c2psi, s2psi);
+    sf.cdc = unif(sf.cdc);
+    sf.sdc = unif(sf.sdc);
+    sf.t0 = unif(sf.t0);
+    sf.c2p = unif(sf.c2p);
+    sf.s2p = unif(sf.s2p);
+    disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
 }
 
 template <int OP>
@@ -204,60 +348,57 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     if (i >= p.s_end)
         return;
 
-    Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
-    c.ra = uni(c.ra);
-    c.dec = uni(c.dec);
-    c.psi = uni(c.psi);
-    c.cdc = uni(c.cdc);
-    c.sdc = uni(c.sdc);
-    double s2psi, c2psi;
-    pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
-    s2psi = uni(s2psi);
-    c2psi = uni(c2psi);
+    Centre c;
+    SampleF sf;
+    SampleD sd;
     Disc d;
-    disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+    sample_setup(p, i, c, sf, sd, d);
     double ga = 0.0, gb = 0.0;
     if (OP != OP_FWD) {
         ga = uni(p.g_a ? (double)p.g_a[i] : 0.0);
         gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
     }
     double acc_a = 0.0, acc_b = 0.0;
+    auto use = [&](const Pending &q) {
+        const float A0 = q.v0.x, A1 = q.v0.y, A2 = q.v0.z;
+        const float B0 = q.v0.w, B1 = q.v1.x, B2 = q.v1.y;
+        const float c2 = q.c2, s2 = q.s2;
+        const int pix = q.pix, bp = q.bp;
+        if (OP == OP_FWD) {
+            float sI = q.sI, sQ = q.sQ, sU = q.sU;
+            float ta = sI * (A0) + sQ * (A1 * c2 - A2 * s2) + sU * (A2 * c2 + A1 * s2);
+            float tb = sI * (B0) + sQ * (-B1 * c2 + B2 * s2) + sU * (-B2 * c2 - B1 * s2);
+            acc_a += ta;
+            acc_b += tb;
+        } else if (OP == OP_ADJ_MAP) {
+            double cc = c2, ss = s2;
+            double vI = ga * A0 + gb * B0;
+            double vQ = ga * (A1 * cc - A2 * ss) + gb * (-B1 * cc + B2 * ss);
+            double vU = ga * (A2 * cc + A1 * ss) + gb * (-B2 * cc - B1 * ss);
+            unsafeAtomicAdd(&p.gI[pix], vI);
+            unsafeAtomicAdd(&p.gQ[pix], vQ);
+            unsafeAtomicAdd(&p.gU[pix], vU);
+        } else {
+            double cc = c2, ss = s2;
+            double qq = p.Q[pix], u = p.U[pix], sI = p.I[pix];
+            double *g = p.gbeam + 8 * (long)bp;
+            unsafeAtomicAdd(&g[0], ga * sI);
+            unsafeAtomicAdd(&g[1], ga * (qq * cc + u * ss));
+            unsafeAtomicAdd(&g[2], ga * (-qq * ss + u * cc));
+            unsafeAtomicAdd(&g[3], gb * sI);
+            unsafeAtomicAdd(&g[4], gb * (-qq * cc - u * ss));
+            unsafeAtomicAdd(&g[5], gb * (qq * ss - u * cc));
+        }
+    };
 
+    int qn = 0;   /* deferred visits (wave-uniform) */
     for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
         int total = build_runs(p.hs, d, r0, lane, t);
-        walk_visits<OP == OP_FWD>(p, t, total, lane, c, c2psi, s2psi, [&](const Pending &q) {
-            const float A0 = q.v0.x, A1 = q.v0.y, A2 = q.v0.z;
-            const float B0 = q.v0.w, B1 = q.v1.x, B2 = q.v1.y;
-            const float c2 = q.c2, s2 = q.s2;
-            const int pix = q.pix, bp = q.bp;
-            if (OP == OP_FWD) {
-                float sI = q.sI, sQ = q.sQ, sU = q.sU;
-                float ta = sI * (A0) + sQ * (A1 * c2 - A2 * s2) + sU * (A2 * c2 + A1 * s2);
-                float tb = sI * (B0) + sQ * (-B1 * c2 + B2 * s2) + sU * (-B2 * c2 - B1 * s2);
-                acc_a += ta;
-                acc_b += tb;
-            } else if (OP == OP_ADJ_MAP) {
-                double cc = c2, ss = s2;
-                double vI = ga * A0 + gb * B0;
-                double vQ = ga * (A1 * cc - A2 * ss) + gb * (-B1 * cc + B2 * ss);
-                double vU = ga * (A2 * cc + A1 * ss) + gb * (-B2 * cc - B1 * ss);
-                unsafeAtomicAdd(&p.gI[pix], vI);
-                unsafeAtomicAdd(&p.gQ[pix], vQ);
-                unsafeAtomicAdd(&p.gU[pix], vU);
-            } else {
-                double cc = c2, ss = s2;
-                double qq = p.Q[pix], u = p.U[pix], sI = p.I[pix];
-                double *g = p.gbeam + 8 * (long)bp;
-                unsafeAtomicAdd(&g[0], ga * sI);
-                unsafeAtomicAdd(&g[1], ga * (qq * cc + u * ss));
-                unsafeAtomicAdd(&g[2], ga * (-qq * ss + u * cc));
-                unsafeAtomicAdd(&g[3], gb * sI);
-                unsafeAtomicAdd(&g[4], gb * (-qq * cc - u * ss));
-                unsafeAtomicAdd(&g[5], gb * (qq * ss - u * cc));
-            }
-        });
+        walk_visits<OP == OP_FWD, OP == OP_FWD>(p, t, total, lane, c, sf, sd, qn, use);
         wave_sync();
     }
+    if (OP != OP_FWD && qn > 0)   /* the forward's mapping never defers */
+        drain_exact<false>(p, t, 0, qn, lane, c, use);
     if (OP == OP_FWD) {
         acc_a = wave_sum(acc_a);
         acc_b = wave_sum(acc_b);
@@ -458,53 +599,48 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
         const int W = fallback ? 0 : ctl[2];
         for (int j = wave; j < ns; j += kAdjWaves) {
             long i = s0 + j;
-            Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
-            c.ra = uni(c.ra);
-            c.dec = uni(c.dec);
-            c.psi = uni(c.psi);
-            c.cdc = uni(c.cdc);
-            c.sdc = uni(c.sdc);
-            double s2psi, c2psi;
-            pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
-            s2psi = uni(s2psi);
-            c2psi = uni(c2psi);
+            Centre c;
+            SampleF sf;
+            SampleD sd;
             Disc d;
-            disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
-            float ga = __builtin_bit_cast(
-                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, p.g_a ? p.g_a[i] : 0.f)));
-            float gb = __builtin_bit_cast(
-                float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, p.g_b ? p.g_b[i] : 0.f)));
+            sample_setup(p, i, c, sf, sd, d);
+            const float ga = unif(p.g_a ? p.g_a[i] : 0.f);
+            const float gb = unif(p.g_b ? p.g_b[i] : 0.f);
+            auto use = [&](const Pending &q) {
+                const int pix = q.pix, r = q.r;
+                float vI, vQ, vU;
+                adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
+                if (vI == 0.f && vQ == 0.f && vU == 0.f)
+                    return;   /* adds nothing */
+                const int k = r - R0;
+                int slot = -1;
+                if (!fallback && (unsigned)k < (unsigned)nring) {
+                    int nr = 4 * (r < p.hs.nside ? r
+                                  : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r : p.hs.nside));
+                    int off = pix - q.sp - win_lo[k];
+                    if (off < 0)
+                        off += nr;
+                    if (off < win_hi[k])   /* inside the ring's window */
+                        slot = win_base[k] + off;
+                }
+                if (slot >= 0) {
+                    atomicAdd(&acc[slot], vI);
+                    atomicAdd(&acc[W + slot], vQ);
+                    atomicAdd(&acc[2 * W + slot], vU);
+                } else {   /* no window holds the pixel: global FP64 atomics */
+                    unsafeAtomicAdd(&p.gI[pix], (double)vI);
+                    unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
+                    unsafeAtomicAdd(&p.gU[pix], (double)vU);
+                }
+            };
+            int qn = 0;
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
                 int total = build_runs(p.hs, d, r0, lane, t);
-                walk_visits<false>(p, t, total, lane, c, c2psi, s2psi, [&](const Pending &q) {
-                    const int pix = q.pix, r = q.r;
-                    float vI, vQ, vU;
-                    adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
-                    if (vI == 0.f && vQ == 0.f && vU == 0.f)
-                        return;   /* adds nothing */
-                    const int k = r - R0;
-                    int slot = -1;
-                    if (!fallback && (unsigned)k < (unsigned)nring) {
-                        int nr = 4 * (r < p.hs.nside ? r
-                                      : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r : p.hs.nside));
-                        int off = pix - q.sp - win_lo[k];
-                        if (off < 0)
-                            off += nr;
-                        if (off < win_hi[k])   /* inside the ring's window */
-                            slot = win_base[k] + off;
-                    }
-                    if (slot >= 0) {
-                        atomicAdd(&acc[slot], vI);
-                        atomicAdd(&acc[W + slot], vQ);
-                        atomicAdd(&acc[2 * W + slot], vU);
-                    } else {   /* no window holds the pixel: global FP64 atomics */
-                        unsafeAtomicAdd(&p.gI[pix], (double)vI);
-                        unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
-                        unsafeAtomicAdd(&p.gU[pix], (double)vU);
-                    }
-                });
+                walk_visits<false, false>(p, t, total, lane, c, sf, sd, qn, use);
                 wave_sync();
             }
+            if (qn > 0)
+                drain_exact<false>(p, t, 0, qn, lane, c, use);
         }
         __syncthreads();
 
@@ -650,11 +786,17 @@ __global__ void k_disc(Hpx h, long n, const float *ra, const float *dec, double 
 }
 
 /* Sky pixel -> (beam pixel, chi) for pixel lists laid out as offsets/pixels
- * (the quantity kernel 1 of the reference computes, cudaConv2.cu:59-128). */
-/* fast != 0: the kernels' certified form (beam_pixel_fast); chi[k] then
- * receives float cos(2chi) and sin(2chi) packed as two floats. */
-__global__ void k_match(Hpx hs, const RingRec *rings, Hpx hb, FastConsts fc, int fast,
-                        long n, const float *ra, const float *dec, const float *pa,
+ * (the quantity kernel 1 of the reference computes, cudaConv2.cu:59-128).
+ * mode 0: the exact FP64 mapping, chi[k] = chi.
+ * mode 1: what the convolution kernels use: the float form, and the exact
+ *         mapping where the float form cannot certify the pixel; chi[k]
+ *         receives float cos(2chi), sin(2chi) packed as two floats.
+ * mode 2: the float form alone; bpix[k] = kNeedExact (-2) where it declines.
+ * mode 3: the certified double-precision form of the forward kernel (output
+ *         as mode 1). */
+__global__ void k_match(Hpx hs, const RingRec *rings, Hpx hb, FastConsts fc, FastConsts64 fc64,
+                        int mode, long n,
+                        const float *ra, const float *dec, const float *pa,
                         const long *offsets, const int *pixels, int *bpix, double *chi)
 {
     long i = (long)blockIdx.x;
@@ -663,14 +805,25 @@ __global__ void k_match(Hpx hs, const RingRec *rings, Hpx hb, FastConsts fc, int
     Centre c = centre_make(ra[i], dec[i], pa[i]);
     double s2psi, c2psi;
     pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+    const SampleF sf = sample_f32(c, c2psi, s2psi);
     for (long k = offsets[i] + threadIdx.x; k < offsets[i + 1]; k += blockDim.x) {
         int pix = pixels[k];
         int r = pix2ring(hs, pix);
         RingRec rr = rings[r];
-        if (fast) {
-            float c2, s2;
+        if (mode > 0) {
+            float c2 = 0.f, s2 = 0.f;
             double ra_pix = ring_phi(hs, rr, r, pix);
-            bpix[k] = beam_pixel_fast(hb, fc, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
+            int bp = (mode == 3)
+                         ? beam_pixel_f64(hb, fc64, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2)
+                         : beam_pixel_f32(fc, c, sf, ra_pix, rr.cdp, rr.sdp, c2, s2);
+            if (bp == kNeedExact && mode == 1) {
+                ExactVisit e =
+                    beam_visit_exact(hb.nside, c.ra, c.psi, c.cdc, c.sdc, ra_pix, rr.cdp, rr.sdp);
+                bp = e.bp;
+                c2 = e.c2;
+                s2 = e.s2;
+            }
+            bpix[k] = bp;
             float2 *o = (float2 *)&chi[k];
             *o = make_float2(c2, s2);
         } else {
@@ -773,6 +926,19 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
     return hipGetLastError();
 }
 
+int adjmap_describe(char *buf, size_t n, int cap, int rw)
+{
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds);
+    if (e != hipSuccess)
+        return snprintf(buf, n, "hipFuncGetAttributes: %s", hipGetErrorString(e));
+    return snprintf(buf, n,
+                    "block %d threads, kernel max %d, regs %d, static LDS %zu, dynamic LDS "
+                    "%zu (max %d), private %zu",
+                    kAdjBlock, a.maxThreadsPerBlock, a.numRegs, a.sharedSizeBytes,
+                    adjmap_lds_bytes(cap, rw), a.maxDynamicSharedSizeBytes, a.localSizeBytes);
+}
+
 hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, last, -1);
@@ -843,24 +1009,37 @@ hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
     return hipGetLastError();
 }
 
-hipError_t launch_match(const Hpx &hs, const RingRec *rings, const Hpx &hb,
-                        const FastConsts *fc, long n, const float *ra, const float *dec,
-                        const float *pa, const long *offsets, const int *pixels, int *bpix,
-                        double *chi, hipStream_t st)
+hipError_t launch_match(const Hpx &hs, const RingRec *rings, const Hpx &hb, int mode, long n,
+                        const float *ra, const float *dec, const float *pa,
+                        const long *offsets, const int *pixels, int *bpix, double *chi,
+                        hipStream_t st)
 {
     if (n <= 0)
         return hipSuccess;
-    FastConsts f = fc ? *fc : fast_consts(hb);
-    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, rings, hb, f,
-                       fc ? 1 : 0, n, ra, dec, pa, offsets, pixels, bpix, chi);
+    hipLaunchKernelGGL(k_match, dim3((unsigned)n), dim3(256), 0, st, hs, rings, hb,
+                       fast_consts(hb), fast_consts64(hb), mode, n, ra, dec, pa, offsets, pixels,
+                       bpix, chi);
     return hipGetLastError();
 }
 
-/* Thresholds of the rho > 1e-6 test on crho (host, with the same libm):
- * rho >= 1.5e-6 is surely above 1e-6 and rho <= 0.5e-6 surely below. */
+/* Constants of beam_pixel_f32 for a beam map.  The rho > 1e-6 band edges are
+ * 1 - cos(x) = x^2/2 (the x^4/24 term is below float resolution). */
 FastConsts fast_consts(const Hpx &hb)
 {
     FastConsts f;
+    f.omc_lo = (float)(0.5 * 0.5e-6 * 0.5e-6);
+    f.omc_hi = (float)(0.5 * 1.5e-6 * 1.5e-6);
+    f.marg_k = (float)(1.2e-13 * hb.nside);
+    f.nside = (float)hb.nside;
+    return f;
+}
+
+/* Constants of beam_pixel_f64: thresholds of the rho > 1e-6 test on crho
+ * (rho >= 1.5e-6 is surely above 1e-6, rho <= 0.5e-6 surely below) and the
+ * floor margin factor. */
+FastConsts64 fast_consts64(const Hpx &hb)
+{
+    FastConsts64 f;
     f.crho_big = pl_cos(1.5e-6);
     f.crho_small = pl_cos(0.5e-6);
     f.marg_k = 100.0 * 4.4e-16 * hb.nside * 1.7320508075688772;

@@ -25,7 +25,8 @@ struct PairParams {
     Hpx hb;
     int npix_beam;
     double rho_max, cos_rmax;
-    FastConsts fc;
+    FastConsts fc;            /* single-precision mapping (sky adjoint, beam adjoint) */
+    FastConsts64 fc64;        /* double-precision mapping (forward) */
     const float *beam;
     /* device int: index of the last beam row whose A0 A1 A2 B0 B1 B2 are not
      * all zero (-1: none), or NULL.  Only sizes the sky adjoint's LDS
@@ -48,6 +49,8 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st);
 size_t adjmap_lds_bytes(int cap, int rw);
+/* Launch resources of the sky-adjoint kernel, for error messages. */
+int adjmap_describe(char *buf, size_t n, int cap, int rw);
 /* *last = index of the last row of beam8 [npix][8] with a non-zero value in
  * columns 0..5 (-1 if none). */
 hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st);
@@ -65,11 +68,12 @@ hipError_t launch_interpol(const Hpx &h, long n, const double *th, const double 
 hipError_t launch_disc(const Hpx &h, long n, const float *ra, const float *dec,
                        double radius, double cosr, const long *offsets, int *counts,
                        int *pixels, hipStream_t st);
-hipError_t launch_match(const Hpx &hs, const RingRec *rings, const Hpx &hb,
-                        const FastConsts *fc, long n, const float *ra, const float *dec,
-                        const float *pa, const long *offsets, const int *pixels, int *bpix,
-                        double *chi, hipStream_t st);
+hipError_t launch_match(const Hpx &hs, const RingRec *rings, const Hpx &hb, int mode, long n,
+                        const float *ra, const float *dec, const float *pa,
+                        const long *offsets, const int *pixels, int *bpix, double *chi,
+                        hipStream_t st);
 FastConsts fast_consts(const Hpx &hb);
+FastConsts64 fast_consts64(const Hpx &hb);
 hipError_t launch_libm(int op, long n, const double *x, const double *y, double *o,
                        hipStream_t st);
 

@@ -91,14 +91,16 @@ class Gpu:
         lib_sync(self)
         return off, host(pix)[:off[-1]]
 
-    def match_fast(self, nside_sky, nside_beam, ra, dec, pa, off, pix):
-        """The kernels' certified fast form: (bpix, cos 2chi, sin 2chi)."""
+    def match_fast(self, nside_sky, nside_beam, ra, dec, pa, off, pix, mode=1):
+        """The kernels' mapping (mode 1: single precision, exact FP64 where it
+        cannot certify; mode 2: single precision alone, bpix -2 where it
+        declines): (bpix, cos 2chi, sin 2chi)."""
         r, d, p = (dev(np.asarray(x, np.float32)) for x in (ra, dec, pa))
         offd = dev(np.asarray(off, np.int64))
         pixd = dev(np.asarray(pix, np.int32))
         bpix = torch.empty(max(pix.size, 1), dtype=torch.int32, device="cuda")
         cs = torch.empty((max(pix.size, 1), 2), dtype=torch.float32, device="cuda")
-        check(lib.pisco_dev_match_mode(self.ctx, None, 1, nside_sky, nside_beam, r.numel(),
+        check(lib.pisco_dev_match_mode(self.ctx, None, mode, nside_sky, nside_beam, r.numel(),
                                        r.data_ptr(), d.data_ptr(), p.data_ptr(),
                                        offd.data_ptr(), pixd.data_ptr(), bpix.data_ptr(),
                                        cs.data_ptr()))

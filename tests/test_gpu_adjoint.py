@@ -54,7 +54,7 @@ def test_adjoint_lds_vs_oracle_and_direct(c3_small, oracle):
     assert np.all(lds[0][sig] != 0)
 
 
-@pytest.mark.parametrize("chunk,cap", [(1, 10240), (7, 10240), (256, 10240), (64, 500)])
+@pytest.mark.parametrize("chunk,cap", [(1, 6144), (7, 6144), (256, 6144), (64, 500)])
 def test_adjoint_lds_shapes_and_fallback(c3_small, oracle, chunk, cap):
     """Chunk sizes (incl. a ragged last chunk) and a window too small for any
     chunk (every chunk takes the direct-atomic fallback)."""

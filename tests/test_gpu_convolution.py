@@ -52,8 +52,11 @@ def test_forward_config1_matches_oracle(c1, oracle):
     _, a, b = run_gpu(128, sky, beam, ra, dec, pa)
     ra_, rb_ = oracle.convolve(128, sky[0], sky[1], sky[2], 1024, beams_of(beam), ra, dec, pa)
     assert rel_rms(a, ra_) < 1e-6 and rel_rms(b, rb_) < 1e-6
-    # with FP64 accumulation in both, almost every sample is bit-identical
-    assert np.mean(a == ra_) > 0.99
+    # per sample: the kernels form cos/sin(2 chi) in float (error <= 2e-6),
+    # which moves a sample by far less than 1e-5 of the TOD's scale
+    scale = np.abs(ra_).max()
+    assert np.max(np.abs(a - ra_)) < 1e-5 * scale
+    assert np.max(np.abs(b - rb_)) < 1e-5 * np.abs(rb_).max()
     da, db = oracle.convolve(128, sky[0], sky[1], sky[2], 1024, beams_of(beam), ra, dec, pa,
                              precision64=True)
     assert rel_rms(a, da) < 1e-6 and rel_rms(b, db) < 1e-6

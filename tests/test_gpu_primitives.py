@@ -133,10 +133,13 @@ def test_get_interpol(gpu, oracle, nside):
                                                           (128, 1024, 2.5), (2048, 8192, 0.3),
                                                           (64, 64, 40.0)])
 def test_match_fast_path_bit_exact(gpu, oracle, nside_sky, nside_beam, cap_deg):
-    """The certified fast form used by the convolution kernels returns exactly
-    the oracle's beam pixels (it falls back to the exact path inside its
-    error margin); cos/sin(2 chi) agree with (float)cos/sin(2 chi) to float
-    rounding."""
+    """The mapping the convolution kernels use (single precision, certified
+    by error margins, exact FP64 where a floor could flip) returns exactly
+    the oracle's beam pixels; cos/sin(2 chi) agree with the oracle's to the
+    single-precision error bound (9.2e-7 measured in float32 emulation;
+    tolerance 2e-6).  The single-precision form alone declines (defers to
+    the exact path) on a small fraction of visits, and every pixel it does
+    certify is the oracle's."""
     rng = np.random.default_rng(17)
     n = 600 if nside_sky <= 512 else 3000
     ra = rng.uniform(0, 2 * math.pi, n).astype(np.float32)
@@ -149,5 +152,17 @@ def test_match_fast_path_bit_exact(gpu, oracle, nside_sky, nside_beam, cap_deg):
     assert np.array_equal(b_h, b_f)
     c2_ref = np.cos(2 * chi_h).astype(np.float32)
     s2_ref = np.sin(2 * chi_h).astype(np.float32)
-    assert np.max(np.abs(c2 - c2_ref)) <= 2.5e-7
-    assert np.max(np.abs(s2 - s2_ref)) <= 2.5e-7
+    assert np.max(np.abs(c2 - c2_ref)) <= 2e-6
+    assert np.max(np.abs(s2 - s2_ref)) <= 2e-6
+    # the forward kernel's certified double-precision form: the same pixels,
+    # cos/sin(2 chi) to float rounding
+    b_3, c3, s3 = gpu.match_fast(nside_sky, nside_beam, ra, dec, pa, off, pix, mode=3)
+    assert np.array_equal(b_h, b_3)
+    assert np.max(np.abs(c3 - c2_ref)) <= 2.5e-7
+    assert np.max(np.abs(s3 - s2_ref)) <= 2.5e-7
+    b_1, _, _ = gpu.match_fast(nside_sky, nside_beam, ra, dec, pa, off, pix, mode=2)
+    cert = b_1 != -2
+    assert np.array_equal(b_1[cert], b_h[cert])
+    declined = 1.0 - cert.mean()
+    print("declined fraction %.2e of %d visits" % (declined, b_1.size))
+    assert declined < 0.01
