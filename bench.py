@@ -21,7 +21,11 @@ with |D_i| the exact disc sizes (device query_disc).  achieved = bytes of
 the dominant kernel / its mean duration (HIP events on the launch stream).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
-                  [--no-cpu] [--forms] [--profile-steps K]
+                  [--no-cpu] [--no-forms] [--calib-copy-mib M]
+
+Also reported: the plain drop-in forward of the NSIDE 256 / 1.5 deg workload
+(device-resident and through the host API including PCIe copies), and the
+CPU oracle on a prefix of the headline workload.
 """
 import argparse
 import ctypes
@@ -35,6 +39,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12   # bytes/s, MI355X
+# HBM-side bytes per launch of the headline workload's kernels, measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy of known size)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_nside512_cap5_4e6.txt")
+
+
+def measured_traffic(samples, kernel):
+    """Calibrated HBM bytes per launch of `kernel` from the committed profile,
+    if it was measured on this sample count; else None."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    vals = dict(l.split(": ", 1) for l in lines if ": " in l and not l.startswith(" "))
+    if vals.get("samples") != str(samples):
+        return None
+    v = vals.get("traffic_bytes_per_launch %s" % kernel)
+    return float(v) if v else None
 
 
 def parse():
@@ -45,8 +66,8 @@ def parse():
     p.add_argument("--samples", type=int, default=4_000_000)
     p.add_argument("--cpu-samples", type=int, default=20_000)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--forms", action="store_true",
-                   help="also time the plain drop-in forward (config 2)")
+    p.add_argument("--no-forms", action="store_true",
+                   help="skip the plain drop-in forward of the NSIDE 256 workload")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="run one device-to-device copy of this size first (a kernel of "
                         "known traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -150,6 +171,7 @@ def main():
 
     dominant = "adjoint_map" if t_adj >= t_fwd else "forward"
     b_dom, t_dom = (b_adj, t_adj) if dominant == "adjoint_map" else (b_fwd, t_fwd)
+    k_dom = "k_adjmap_lds" if dominant == "adjoint_map" else "k_pair<0>"
     achieved = b_dom / t_dom
     out = {
         "metric": "TOD samples/s fwd+adjoint @NSIDE512 5deg beam (pair); %HBM peak",
@@ -170,20 +192,89 @@ def main():
                    "samples": n, "detectors": k, "visits_per_pass": visits,
                    "mean_disc": visits / n},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": ("k_adjmap_lds" if dominant == "adjoint_map"
-                                else "k_pair<OP_FWD>"),
-                     "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(n, k_dom),
+                     "kernel": k_dom, "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
         "fwd_adj": {"forward_s": t_fwd, "adjoint_s": t_adj,
                     "frac_hbm_fwd_plus_adj": (b_fwd + b_adj) / (t_fwd + t_adj) / HBM_PEAK,
                     "forward_tod_samples_per_s": k * n / t_fwd,
                     "wall_s_timed": wall, "setup_s": setup_s},
     }
 
+    if not args.no_forms:
+        out["plain_forward_nside256"] = plain_forward(ctx, stream, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra,
                                            dec, pa, g)
     print(json.dumps(out), flush=True)
+
+
+def plain_forward(ctx, stream, dev, n=1_000_000, reps=5):
+    """The plain drop-in forward of a detector pair: sky NSIDE 256 (power-law,
+    l <= 256), beam NSIDE 2048, cap 1.5 deg, FWHM 0.5 deg, 1e6 CES samples.
+    Timed two ways: device-resident (inputs in HBM, HIP events) and through
+    the host API the reference exposes (GPUConvolver.update_scan +
+    exec_convolution with host arrays: includes the PCIe copies)."""
+    import numpy as np
+    import torch
+
+    from pisco4cmb_amd import inputs, pypisco, torch_ops
+    from pisco4cmb_amd._lib import check, lib
+    nside_sky, nside_beam = 256, 2048
+    I, Q, U, _ = inputs.powerlaw_sky(nside_sky, seed=77, lmax=256)
+    beam = inputs.gaussian_polbeam(nside_beam, 1.5, 0.5, 0.0, 0.0, nside_sky=nside_sky)
+    ra, dec, pa = inputs.ces_scan(n)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    It, Qt, Ut, rat, dect, pat = (t(x) for x in (I, Q, U, ra, dec, pa))
+    beam8 = torch_ops.pack_beam(t(np.stack(beam.aBeams)), t(np.stack(beam.bBeams)))
+    npb = beam8.shape[0]
+    ta = torch.empty(n, dtype=torch.float32, device=dev)
+    tb = torch.empty_like(ta)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    check(lib.pisco_dev_query_disc(ctx, stream, nside_sky, n, rat.data_ptr(), dect.data_ptr(),
+                                   lib.pisco_beam_rho_max(nside_beam, npb), None,
+                                   counts.data_ptr(), None))
+    torch.cuda.synchronize()
+    visits = int(counts.to(torch.int64).sum().item())
+
+    def fwd():
+        check(lib.pisco_dev_forward(ctx, stream, nside_sky, It.data_ptr(), Qt.data_ptr(),
+                                    Ut.data_ptr(), nside_beam, npb, beam8.data_ptr(), n,
+                                    rat.data_ptr(), dect.data_ptr(), pat.data_ptr(),
+                                    ta.data_ptr(), tb.data_ptr()))
+    fwd()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fwd()
+    e1.record()
+    torch.cuda.synchronize()
+    t_dev = e0.elapsed_time(e1) / reps / 1e3
+    b = visits * 36 + n * 20
+
+    conv = pypisco.GPUConvolver(pypisco.default_run_config(ptgPerConv=n))
+    conv.configure_execution()
+    conv.update_sky(pypisco.Sky(nside_sky, I, Q, U))
+    conv.update_beam(beam)
+    da = np.zeros(n, np.float32)
+    db = np.zeros(n, np.float32)
+    scan = pypisco.Scan(n, ra, dec, pa)
+    conv.update_scan(scan)
+    conv.exec_convolution(da, db)
+    w0 = time.time()
+    for _ in range(reps):
+        conv.update_scan(scan)
+        conv.exec_convolution(da, db)
+    t_host = (time.time() - w0) / reps
+    conv.close()
+    return {"workload": "forward, sky NSIDE 256, pair, FWHM 0.5 deg, cap 1.5 deg, beam "
+                        "NSIDE 2048, %d samples whole-sky CES" % n,
+            "visits": visits, "mean_disc": visits / n,
+            "device_resident": {"tod_samples_per_s": 2 * n / t_dev, "s": t_dev,
+                                "hbm_frac": b / t_dev / HBM_PEAK},
+            "host_api_with_pcie": {"tod_samples_per_s": 2 * n / t_host, "s": t_host}}
 
 
 def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g):

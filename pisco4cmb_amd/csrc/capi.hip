@@ -104,7 +104,7 @@ struct pisco_ctx {
     std::map<int, RingRec *> ring_tabs;
     int ncu = 256;                /* compute units of the device */
     /* tuning (pisco_set_tuning) */
-    int adjmap_chunk = 64;        /* samples per LDS window; 0 = direct atomics */
+    int adjmap_chunk = 16;        /* samples per LDS window; 0 = direct atomics */
     int adjmap_cap = 6144;        /* LDS window pixels */
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
