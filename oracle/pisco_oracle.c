@@ -31,7 +31,21 @@
  * the test suite compares the two builds (index mismatch rate) as a
  * diagnostic of how far the shared library moves results from glibc.
  *
- * Parity status of each part (how it is pinned) is documented in DESIGN.md.
+ * How each part is pinned (tests/ under the CPU marker):
+ *   pix2ang / ang2pix  round trip of every pixel (NSIDE <= 256, sampled 4096
+ *                      and 8192), closed-form ring z and phi, published NSIDE 1
+ *                      values, equal-area sampling by brute force
+ *   query_disc         brute force over all pixels (NSIDE <= 64), area law
+ *   get_interpol       weights sum to one, non-negative, exact at pixel centres
+ *   rho_sigma_chi      bit-exact against golden vectors written by the
+ *                      reference's own sphtrigo.cpp (tests/golden), an
+ *                      independent 3-vector construction, on-axis closed forms
+ *   Convolver forward  point source traces the beam, uniform sky sums the
+ *                      beam, linearity, a/b antisymmetry, 90-degree rotation
+ *   PolBeam            closed form of the elliptical Gaussian
+ *   adjoints           dot-product identity against the forward
+ * The TOD sums and query_disc have no golden values in the reference; beyond
+ * these properties their bits are not pinned against the original build.
  * Compile with -ffp-contract=off (see Makefile).
  */
 #include <math.h>

@@ -92,7 +92,7 @@ def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libpisco.so is not built (%s); run `python -m pisco4cmb_amd.build` "
-            "(or __graft_entry__.build()) first" % LIB_PATH)
+            "first" % LIB_PATH)
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

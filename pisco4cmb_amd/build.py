@@ -1,6 +1,6 @@
 """Build the gfx950 shared library libpisco.so in-tree with hipcc.
 
-    python -m pisco4cmb_amd.build            # library only
+    python -m pisco4cmb_amd.build            # library (+ the tests' C++ driver)
     python -m pisco4cmb_amd.build --oracle   # also the CPU oracle (make -C oracle)
 
 The library is the product: hand-written HIP kernels for gfx950 and the
@@ -51,6 +51,25 @@ def build_library(verbose=False):
     return LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "drop_in_driver")
+
+
+def build_cpp_driver(verbose=False):
+    """The C++ program of the test suite that uses include/pisco.hpp the way a
+    reference program uses PISCO's classes (plain g++, links libpisco)."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L" + HERE, "-lpisco", "-Wl,-rpath,$ORIGIN/../../pisco4cmb_amd", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building the C++ drop-in driver failed")
+    return DRIVER
+
+
 def build_oracle(verbose=False):
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-B"],
                        capture_output=True, text=True)
@@ -64,6 +83,7 @@ def build_oracle(verbose=False):
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     build_library(verbose=True)
+    build_cpp_driver(verbose=True)
     if "--oracle" in argv:
         build_oracle(verbose=True)
 

@@ -182,3 +182,29 @@ def test_shared_libm_vs_glibc_index_diagnostic(oracle, oracle_glibc):
     mism = int(np.sum(b1 != b2))
     print("beam-pixel mismatches shared-libm vs glibc: %d of %d" % (mism, b1.size))
     assert mism <= 1e-5 * b1.size
+
+
+def test_adjoints_satisfy_the_dot_product_identity(oracle, setup):
+    """The oracle's adjoints are the transposes of its forward:
+    <F x, g> = <x, F^T g> for the sky (I,Q,U) and for the beams (A0..A2,
+    B0..B2; the forward does not read V).  The adjoints use cos/sin(2 chi)
+    rounded to float as the reference's forward does, the FP64 forward keeps
+    them in double: the two sides differ by that rounding (~1e-8 relative),
+    while a dropped term, a wrong sign or a transposed operand would move
+    them by O(0.1-1).  Tolerance 1e-6 relative."""
+    nside_sky, nside_beam, beam, ra, dec, pa = setup
+    rng = np.random.default_rng(11)
+    npix = 12 * nside_sky ** 2
+    I, Q, U = (rng.normal(size=npix).astype(np.float32) for _ in range(3))
+    ga, gb = (rng.normal(size=ra.size).astype(np.float32) for _ in range(2))
+    beams = beams_of(beam)
+    da, db = oracle.convolve(nside_sky, I, Q, U, nside_beam, beams, ra, dec, pa,
+                             precision64=True)
+    lhs = float(np.dot(da, ga) + np.dot(db, gb))
+    gI, gQ, gU = oracle.adjoint_map(nside_sky, nside_beam, beams, ra, dec, pa, ga, gb)
+    rhs_sky = float(np.dot(gI, I) + np.dot(gQ, Q) + np.dot(gU, U))
+    assert abs(lhs - rhs_sky) <= 1e-6 * abs(lhs)
+    G = oracle.adjoint_beam(nside_sky, I, Q, U, nside_beam, beams, ra, dec, pa, ga, gb)
+    used = [beams[0], beams[1], beams[2], beams[4], beams[5], beams[6]]
+    rhs_beam = float(sum(np.dot(G[:, k], used[k].astype(np.float64)) for k in range(6)))
+    assert abs(lhs - rhs_beam) <= 1e-6 * abs(lhs)
