@@ -119,7 +119,8 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * 0 selects direct global atomics), "adjmap_cap" (window pixels),
  * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
  * per CU), "adjmap_support" (1: windows cover only the discs of the beam's
- * non-zero rows, 0: the full rho_max discs; results are the same either way);
+ * non-zero rows, 0: the full rho_max discs; results are the same either way),
+ * "batch_group" (detector pairs per wave in batched launches: 2 or 4);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);
@@ -153,6 +154,37 @@ int pisco_dev_adjoint_beam(pisco_ctx *ctx, void *stream, int nside_sky,
                            int nside_beam, long npix_beam, long nsamples,
                            const float *ra, const float *dec, const float *pa,
                            const float *g_a, const float *g_b, float *gbeam8);
+
+/* ---- batched detector pairs (device-resident) --------------------------- */
+
+/* One detector pair of a batched launch: its beam (beam8 layout, device), its
+ * beam map geometry and the offset of its position angle from the shared
+ * pointing: the pair's pa is (float)(pa + pa_offset), i.e. what its own Scan
+ * would hold.  The disc radius is rhoMax of its beam (polbeam.cpp:35-36). */
+typedef struct pisco_batch_pair {
+    int nside_beam;
+    long npix_beam;
+    const float *beam8;
+    float pa_offset;
+} pisco_batch_pair;
+
+typedef struct pisco_batch pisco_batch;
+
+/* A detector table on the context's device (no counterpart in the reference,
+ * whose GPUConvolver holds one pair: gpuconvolver.hpp:17-82).  The table is
+ * copied; the beams are referenced (they must stay valid while in use). */
+int pisco_batch_create(pisco_ctx *ctx, int npairs, const pisco_batch_pair *pairs,
+                       pisco_batch **out);
+void pisco_batch_destroy(pisco_batch *batch);
+
+/* Forward of every pair of the batch in one launch: tod (device) receives
+ * npairs x 2 x nsamples floats, tod[(2k + 0) nsamples + i] = detector a of
+ * pair k at sample i, [(2k + 1) nsamples + i] = detector b.  Each pair's TOD
+ * is what pisco_dev_forward gives for that pair with pa + pa_offset. */
+int pisco_dev_batched_forward(pisco_ctx *ctx, void *stream, const pisco_batch *batch,
+                              int nside_sky, const float *I, const float *Q,
+                              const float *U, long nsamples, const float *ra,
+                              const float *dec, const float *pa, float *tod);
 
 /* ---- HEALPix / matching primitives on the device (for verification) ---- */
 

@@ -39,6 +39,12 @@ class RunConfig(ctypes.Structure):
                 ("blockSizeY", c_int), ("ptgPerConv", c_int), ("pixelsPerDisc", c_int)]
 
 
+class BatchPair(ctypes.Structure):
+    """pisco_batch_pair (include/pisco.h)."""
+    _fields_ = [("nside_beam", c_int), ("npix_beam", c_long), ("beam8", c_void_p),
+                ("pa_offset", ctypes.c_float)]
+
+
 # name -> (restype, argtypes); mirrors include/pisco.h one to one
 SIGNATURES = {
     "pisco_last_error": (ctypes.c_char_p, []),
@@ -60,6 +66,11 @@ SIGNATURES = {
                                       P, P, P, P, P, P, P, P]),
     "pisco_dev_adjoint_beam": (c_int, [c_void_p, c_void_p, c_int, P, P, P, c_int, c_long,
                                        c_long, P, P, P, P, P, P]),
+    "pisco_batch_create": (c_int, [c_void_p, c_int, ctypes.POINTER(BatchPair),
+                                   ctypes.POINTER(c_void_p)]),
+    "pisco_batch_destroy": (None, [c_void_p]),
+    "pisco_dev_batched_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P, P, P, c_long,
+                                          P, P, P, P]),
     "pisco_dev_pix2ang": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_ang2pix": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_get_interpol": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P, P]),

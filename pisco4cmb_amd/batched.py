@@ -1,0 +1,67 @@
+"""Batched detector pairs on the device: many PolBeam pairs that share one
+boresight pointing (each with its own beam, disc radius and position-angle
+offset) convolved in one launch (pisco_batch_create /
+pisco_dev_batched_forward, include/pisco.h).
+
+    batch = DetectorBatch([(beam_a, beam_b, nside_beam, pa_offset), ...])
+    tod = batch.forward(I, Q, U, ra, dec, pa, nside_sky)   # [npairs, 2, n]
+
+beam_a, beam_b: float32 [4, npix] cuda tensors (PolBeam aBeams / bBeams);
+pair k's TOD is what the single-pair forward gives with pa + pa_offset (in
+float).  torch supplies device memory and the current stream only.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .torch_ops import _ctx, _req, _stream, pack_beam
+
+
+class DetectorBatch:
+    def __init__(self, pairs, device=None):
+        if not pairs:
+            raise ValueError("a batch needs at least one pair")
+        self.device = device or pairs[0][0].device
+        self.beam8 = []
+        table = (_lib.BatchPair * len(pairs))()
+        for k, (ba, bb, nside_beam, pa_off) in enumerate(pairs):
+            b8 = pack_beam(ba, bb)
+            self.beam8.append(b8)   # the batch references the packed rows
+            table[k].nside_beam = int(nside_beam)
+            table[k].npix_beam = int(b8.shape[0])
+            table[k].beam8 = b8.data_ptr()
+            table[k].pa_offset = float(pa_off)
+        self.npairs = len(pairs)
+        self.nside_beams = [int(p[2]) for p in pairs]
+        self.ctx = _ctx(self.device)
+        torch.cuda.synchronize(self.device)   # packed rows complete before the table refers to them
+        h = ctypes.c_void_p()
+        check(lib.pisco_batch_create(self.ctx, self.npairs, table, ctypes.byref(h)))
+        self.handle = h
+
+    def forward(self, I, Q, U, ra, dec, pa, nside_sky):
+        """TOD of every pair: float32 [npairs, 2, nsamples] on the device."""
+        I, Q, U = (_req(x, n) for x, n in ((I, "I"), (Q, "Q"), (U, "U")))
+        ra, dec, pa = (_req(x, n) for x, n in ((ra, "ra"), (dec, "dec"), (pa, "pa")))
+        if I.numel() != 12 * nside_sky * nside_sky:
+            raise ValueError("sky maps must have 12*nside_sky^2 pixels")
+        n = ra.numel()
+        tod = torch.empty((self.npairs, 2, n), dtype=torch.float32, device=self.device)
+        check(lib.pisco_dev_batched_forward(self.ctx, _stream(self.device), self.handle,
+                                            nside_sky, I.data_ptr(), Q.data_ptr(),
+                                            U.data_ptr(), n, ra.data_ptr(), dec.data_ptr(),
+                                            pa.data_ptr(), tod.data_ptr()))
+        return tod
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.pisco_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

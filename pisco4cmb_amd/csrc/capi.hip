@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/pisco.h"
 #include "device_hpx.h"
@@ -109,6 +110,7 @@ struct pisco_ctx {
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
+    int batch_group = 2;          /* detector pairs per wave in batched launches */
     DevBuf<int> support;          /* last non-zero beam row (device) */
 };
 
@@ -275,6 +277,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 0 && value != 1)
             return fail(PISCO_ERR_INVALID, "adjmap_support must be 0 or 1");
         c->adjmap_support = (int)value;
+    } else if (k == "batch_group") {
+        if (value != 2 && value != 4)
+            return fail(PISCO_ERR_INVALID, "batch_group must be 2 or 4");
+        c->batch_group = (int)value;
     } else if (k == "adjmap_blocks") {
         if (value < 0 || value > 65536)
             return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
@@ -300,6 +306,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjmap_blocks;
     else if (k == "adjmap_support")
         *value = c->adjmap_support;
+    else if (k == "batch_group")
+        *value = c->batch_group;
     else if (k == "cu_count")
         *value = c->ncu;
     else
@@ -843,3 +851,122 @@ int pisco_dev_libm(pisco_ctx *c, void *stream, int op, long n, const double *x,
 }
 
 } /* extern "C" */
+
+/* ---- batched detector pairs ---------------------------------------------- */
+
+struct pisco_batch {
+    int device = 0;
+    int npairs = 0;
+    int npad = 0;                 /* table length: npairs rounded up to 4 */
+    BatchPair *table = nullptr;   /* device, sorted by rho_max descending */
+    std::vector<BatchPair> host;  /* same, on the host */
+};
+
+int pisco_batch_create(pisco_ctx *c, int npairs, const pisco_batch_pair *pairs,
+                       pisco_batch **out)
+{
+    if (!c || !pairs || !out || npairs < 1)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    *out = nullptr;
+    std::vector<BatchPair> t((size_t)npairs);
+    for (int k = 0; k < npairs; k++) {
+        const pisco_batch_pair &q = pairs[k];
+        if (!valid_nside(q.nside_beam))
+            return fail(PISCO_ERR_INVALID, "pair %d: NSIDE out of range", k);
+        long npb_max = 12L * q.nside_beam * q.nside_beam;
+        if (q.npix_beam < 1 || q.npix_beam >= npb_max || !q.beam8)
+            return fail(PISCO_ERR_INVALID, "pair %d: bad beam", k);
+        BatchPair &b = t[(size_t)k];
+        memset(&b, 0, sizeof b);
+        b.beam = q.beam8;
+        b.npix = (int)q.npix_beam;
+        b.nside = q.nside_beam;
+        FastConsts fc = fast_consts(hpx_make(q.nside_beam));
+        b.nside_f = fc.nside;
+        b.marg_k = fc.marg_k;
+        b.rho_max = pisco_beam_rho_max(q.nside_beam, q.npix_beam);
+        b.cos_rmax = cos_radius(b.rho_max);
+        const double sh = pl_sin(0.5 * b.rho_max);
+        b.omc_cut = (float)(2.0 * sh * sh);   /* 1 - cos(rho_max) without cancellation */
+        b.pa_off = q.pa_offset;
+        b.out = k;
+    }
+    std::stable_sort(t.begin(), t.end(),
+                     [](const BatchPair &a, const BatchPair &b) { return a.rho_max > b.rho_max; });
+    const int npad = (npairs + 3) / 4 * 4;
+    while ((int)t.size() < npad) {   /* inert pairs: never members, no output */
+        BatchPair b = t[0];
+        b.npix = 0;
+        b.omc_cut = -1.0f;
+        b.rho_max = 0.0;
+        b.cos_rmax = 1.0;
+        b.out = -1;
+        t.push_back(b);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    BatchPair *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, sizeof(BatchPair) * t.size()));
+    hipError_t e = hipMemcpy(d, t.data(), sizeof(BatchPair) * t.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(PISCO_ERR_DEVICE, "hipMemcpy failed: %s", hipGetErrorString(e));
+    }
+    pisco_batch *bt = new pisco_batch();
+    bt->device = c->device;
+    bt->npairs = npairs;
+    bt->npad = npad;
+    bt->table = d;
+    bt->host = std::move(t);
+    *out = bt;
+    return PISCO_OK;
+}
+
+void pisco_batch_destroy(pisco_batch *b)
+{
+    if (!b)
+        return;
+    (void)hipSetDevice(b->device);
+    (void)hipFree(b->table);
+    delete b;
+}
+
+int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, int nside_sky,
+                              const float *I, const float *Q, const float *U, long nsamples,
+                              const float *ra, const float *dec, const float *pa, float *tod)
+{
+    if (!c || !b || !I || !Q || !U || !tod || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (!valid_nside(nside_sky) || nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (b->device != c->device)
+        return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    HIPCHK(hipSetDevice(c->device));
+    const RingRec *rt = nullptr;
+    CHK(ring_table(c, nside_sky, &rt));
+    BatchParams p;
+    memset(&p, 0, sizeof p);
+    p.hs = hpx_make(nside_sky);
+    p.rings = rt;
+    p.I = I;
+    p.Q = Q;
+    p.U = U;
+    p.pairs = b->table;
+    p.npairs = b->npad;
+    const int G = c->batch_group;
+    p.ngroups = b->npad / G;
+    p.fc = fast_consts(hpx_make(1));   /* only the rho > 1e-6 band is read */
+    p.ra = ra;
+    p.dec = dec;
+    p.pa = pa;
+    p.nsamples = nsamples;
+    p.tod = tod;
+    hipStream_t st = pick(c, stream);
+    long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : nsamples;
+    chunk = std::max<long>(chunk, std::min<long>(nsamples, 1L << 20));
+    for (long s0 = 0; s0 < nsamples; s0 += chunk) {
+        p.s_begin = s0;
+        p.s_end = std::min(nsamples, s0 + chunk);
+        HIPCHK(launch_batch_forward(G, p, st));
+    }
+    return PISCO_OK;
+}

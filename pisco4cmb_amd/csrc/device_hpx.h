@@ -560,57 +560,88 @@ __device__ inline VisitRing visit_ring_f32(const Centre &c, double ra_pix, doubl
     return g;
 }
 
-/* Beam pixel and float cos/sin 2chi of a visit, or kNeedExact (c2, s2 then
- * unset). */
-__device__ inline int beam_pixel_f32_tail(const FastConsts &fc, const SampleF &sf,
-                                          const VisitRing &g, float &c2, float &s2)
+/* The part of the float mapping that does not depend on the detector's
+ * position angle: 1 - cos rho, alpha, and e^{2i(alpha+beta)} (er, ei). */
+struct VisitGeo {
+    float omc;        /* 1 - cos rho */
+    float x3y;        /* sqrt(3 (1 - cos rho)) */
+    float y;          /* 1/sqrt(3 (1 - cos rho)) */
+    float alpha;
+    float er, ei;     /* e^{2i(alpha+beta)} */
+    bool exact;       /* rho within 0.5e-6 of 1e-6, or degenerate: exact path */
+};
+
+__device__ inline VisitGeo visit_geo_f32(const FastConsts &fc, const SampleF &sf,
+                                         const VisitRing &g)
 {
+    VisitGeo v;
     float sh, ch;
     plf_sincos(g.xh, &sh, &ch);
     const float sdra = 2.0f * (sh * ch);
     const float h2 = 2.0f * (sh * sh);
     const float omD = (g.sD * g.sD) * __builtin_amdgcn_rcpf(g.opcD);
-    const float omc = fmaf(h2, sf.cdc * g.cdp, omD);   /* 1 - cos rho */
+    v.omc = fmaf(h2, sf.cdc * g.cdp, omD);
     const float ya = sdra * g.cdp, xa = fmaf(h2, g.cdp * sf.sdc, g.sD);
     const float yb = sdra * sf.cdc, xb = fmaf(h2, sf.cdc * g.sdp, -g.sD);
-
-    const bool big = omc > fc.omc_hi;
-    if (!big && !(omc < fc.omc_lo))
-        return kNeedExact;   /* rho within 0.5e-6 of 1e-6 */
-    float alpha = 0x1.921fb6p+0f;   /* pi/2 when rho <= 1e-6 */
-    float er = 1.0f, ei = 0.0f;
+    const bool big = v.omc > fc.omc_hi;
+    v.exact = !big && !(v.omc < fc.omc_lo);   /* rho within 0.5e-6 of 1e-6 */
+    v.alpha = 0x1.921fb6p+0f;                 /* pi/2 when rho <= 1e-6 */
+    v.er = 1.0f;
+    v.ei = 0.0f;
     if (big) {
-        alpha = plf_atan2_core(ya, xa, __builtin_amdgcn_rcpf(fmaxf(fabsf(xa), fabsf(ya))));
+        v.alpha = plf_atan2_core(ya, xa, __builtin_amdgcn_rcpf(fmaxf(fabsf(xa), fabsf(ya))));
         const float wr = xb * xa - yb * ya, wi = xb * ya + yb * xa;
         const float n = wr * wr + wi * wi;
         if (!(n > 1e-36f))
-            return kNeedExact;   /* |w|^2 >= 5e-24 for rho > 1.5e-6 */
+            v.exact = true;   /* |w|^2 >= 5e-24 for rho > 1.5e-6 */
         const float inv = __builtin_amdgcn_rcpf(n);
-        er = (wr * wr - wi * wi) * inv;
-        ei = 2.0f * (wr * wi) * inv;
+        v.er = (wr * wr - wi * wi) * inv;
+        v.ei = 2.0f * (wr * wi) * inv;
     }
-    c2 = er * sf.c2p - ei * sf.s2p;
-    s2 = er * sf.s2p + ei * sf.c2p;
+    const float x3 = 3.0f * v.omc;
+    v.y = __builtin_amdgcn_rsqf(x3);
+    v.x3y = x3 * v.y;
+    return v;
+}
 
-    float tt = fmaf(-alpha, 0x1.45f306p-1f /* 2/pi */, sf.t0);
+/* The detector's part: its beam pixel (beam map of NSIDE nside with margin
+ * factor marg_k) and float cos/sin 2chi, given t0 = (pi - psi) 2/pi mod 4 and
+ * cos/sin 2psi of its position angle; kNeedExact where a floor is not
+ * certain. */
+__device__ inline int beam_pixel_f32_det(const VisitGeo &v, float t0, float c2p, float s2p,
+                                         float nside, float marg_k, float &c2, float &s2)
+{
+    c2 = v.er * c2p - v.ei * s2p;
+    s2 = v.er * s2p + v.ei * c2p;
+    float tt = fmaf(-v.alpha, 0x1.45f306p-1f /* 2/pi */, t0);
     tt = (tt < 0.0f) ? tt + 4.0f : tt;
     tt = (tt >= 4.0f) ? tt - 4.0f : tt;
-    const float x3 = 3.0f * omc;
-    const float y = __builtin_amdgcn_rsqf(x3);   /* 1/sqrt(3 (1 - cos rho)) */
-    const float tmp = fc.nside * (x3 * y);
+    const float tmp = nside * v.x3y;
     const float tp = tt - (float)(int)tt;
     const float u1 = tp * tmp, u2 = (1.0f - tp) * tmp;
     const int jp = (int)u1, jm = (int)u2;
     const float d1 = u1 - (float)jp, d2 = u2 - (float)jm;
-    const float md = fmaf(tmp, 8e-6f, fmaf(fc.marg_k, y, 1e-6f));
+    const float md = fmaf(tmp, 8e-6f, fmaf(marg_k, v.y, 1e-6f));
     const int ir = jp + jm + 1;
-    const float v = tt * (float)ir;
-    const int ip = (int)v;
-    const float dv = v - (float)ip;
+    const float vv = tt * (float)ir;
+    const int ip = (int)vv;
+    const float dv = vv - (float)ip;
     const float mv = fmaf((float)ir, 4.5e-6f, 1e-6f);
-    const bool ok = (omc > 0.0f) && (omc < 0.33332f) && (d1 > md) && (d1 < 1.0f - md) &&
-                    (d2 > md) && (d2 < 1.0f - md) && (dv > mv) && (dv < 1.0f - mv);
+    const bool ok = !v.exact && (v.omc > 0.0f) && (v.omc < 0.33332f) && (d1 > md) &&
+                    (d1 < 1.0f - md) && (d2 > md) && (d2 < 1.0f - md) && (dv > mv) &&
+                    (dv < 1.0f - mv);
     return ok ? 2 * ir * (ir - 1) + ip : kNeedExact;
+}
+
+/* Beam pixel and float cos/sin 2chi of a visit, or kNeedExact (c2, s2 then
+ * unset). */
+__device__ inline int beam_pixel_f32_tail(const FastConsts &fc, const SampleF &sf,
+                                          const VisitRing &g, float &c2, float &s2)
+{
+    const VisitGeo v = visit_geo_f32(fc, sf, g);
+    if (v.exact)
+        return kNeedExact;
+    return beam_pixel_f32_det(v, sf.t0, sf.c2p, sf.s2p, fc.nside, fc.marg_k, c2, s2);
 }
 
 /* Beam pixel and float cos/sin 2chi of the visit of the sky pixel at

@@ -411,6 +411,261 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     }
 }
 
+/* ---- batched forward: many detector pairs on one pointing ----------------
+ *
+ * Detector pairs that share the boresight pointing (each with its own beam,
+ * disc radius and position-angle offset) are convolved in one launch.  A wave
+ * takes one (sample, group of G pairs); the pairs are sorted by disc radius,
+ * so the group's union disc is the disc of its first pair.  Lanes walk the
+ * visits of that disc as in k_pair; per visit the parts that do not depend on
+ * the pair (sky values, the float geometry: 1 - cos rho, alpha, the chi
+ * phase) are computed once and each pair runs only its tail: disc membership
+ * (its own rho_max), beam pixel for its position angle, beam gather, sum.
+ * Membership is decided in float away from the disc edge and by the exact
+ * query_disc ring test (disc_ring at the pair's radius) within 4e-6 of it;
+ * like uncertain beam pixels, such visits go to the per-wave queue and are
+ * evaluated exactly, their sums kept in LDS.  The pair table is padded to a
+ * multiple of G with inert pairs (npix 0, never members). */
+
+constexpr int kMaxGroup = 4;
+constexpr int kBatchQueue = (kMaxGroup + 1) * kWaveSize;   /* one visit adds <= G x 64 */
+
+struct BatchTable {
+    RunTable rt;                    /* run table (its own queue is not used) */
+    int qpix[kBatchQueue];          /* queue: sky pixel, ring, pair in the group */
+    int qring[kBatchQueue];
+    int qpair[kBatchQueue];
+    double acc[kMaxGroup][2];       /* exact-path sums of each pair */
+    double psi[kMaxGroup];          /* psi of each pair at this sample */
+};
+
+/* Exact contribution (forward) of a queued visit to pair d: membership by
+ * the query_disc ring test at the pair's radius, then the exact mapping. */
+__device__ inline bool batch_exact_visit(const BatchParams &p, const BatchPair &bp_,
+                                         const Centre &c, double psi, int pix, int r,
+                                         float &ta, float &tb)
+{
+    Disc dk;
+    disc_setup(p.hs, kHalfPi - c.dec, c.ra, bp_.rho_max, bp_.cos_rmax, dk);
+    if (r < dk.r_first || r > dk.r_last)
+        return false;
+    int p0[2] = {0, 0}, ln[2] = {0, 0};
+    const int nrun = disc_ring(p.hs, dk, r, p0, ln);
+    bool in = false;
+    for (int k = 0; k < nrun; k++)
+        in = in || ((unsigned)(pix - p0[k]) < (unsigned)ln[k]);
+    if (!in)
+        return false;
+    const RingRec rr = p.rings[r];
+    const double ra_pix = ring_phi(p.hs, rr, r, pix);
+    const ExactVisit e =
+        beam_visit_exact(bp_.nside, c.ra, psi, c.cdc, c.sdc, ra_pix, rr.cdp, rr.sdp);
+    if ((unsigned)e.bp >= (unsigned)bp_.npix)
+        return false;
+    const float *row = bp_.beam + 8 * (long)e.bp;
+    const float4 v0 = *(const float4 *)row;
+    const float2 v1 = *(const float2 *)(row + 4);
+    const float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
+    ta = sI * v0.x + sQ * (v0.y * e.c2 - v0.z * e.s2) + sU * (v0.z * e.c2 + v0.y * e.s2);
+    tb = sI * v0.w + sQ * (-v1.x * e.c2 + v1.y * e.s2) + sU * (-v1.y * e.c2 - v1.x * e.s2);
+    return true;
+}
+
+template <int G>
+__device__ inline void batch_drain(const BatchParams &p, BatchTable &bt, int d0, int base,
+                                   int cnt, int lane, const Centre &c)
+{
+    wave_sync();
+    if (lane < cnt) {
+        const int pix = bt.qpix[base + lane], r = bt.qring[base + lane];
+        const int k = bt.qpair[base + lane];
+        float ta, tb;
+        if (batch_exact_visit(p, p.pairs[d0 + k], c, bt.psi[k], pix, r, ta, tb)) {
+            atomicAdd(&bt.acc[k][0], (double)ta);
+            atomicAdd(&bt.acc[k][1], (double)tb);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_batch_fwd(BatchParams p)
+{
+    __shared__ BatchTable tabs[kWavesPerBlock];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
+    const int lane = threadIdx.x % kWaveSize;
+    BatchTable &bt = tabs[wave];
+    RunTable &t = bt.rt;
+
+    const long lb = xcd_remap(blockIdx.x, gridDim.x);
+    const long w = lb * kWavesPerBlock + wave;
+    const long nsamp = p.s_end - p.s_begin;
+    if (w >= nsamp * p.ngroups)
+        return;
+    const long i = p.s_begin + w / p.ngroups;
+    const int d0 = (int)(w % p.ngroups) * G;
+
+    Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+    c.ra = uni(c.ra);
+    c.dec = uni(c.dec);
+    c.cdc = uni(c.cdc);
+    c.sdc = uni(c.sdc);
+    SampleF sf;
+    sf.cdc = unif((float)c.cdc);
+    sf.sdc = unif((float)c.sdc);
+
+    /* per-pair constants of this sample: lane k computes pair d0 + k */
+    float t0l = 0.f, c2l = 1.f, s2l = 0.f;
+    if (lane < G) {
+        const float pa_k = p.pa[i] + p.pairs[d0 + lane].pa_off;   /* the pair's own pa */
+        const double psi = -(double)pa_k;
+        double s2, c2;
+        pl_sincos(2.0 * psi, &s2, &c2);
+        t0l = (float)fmodulo((kPi - psi) * kInvHalfPi, 4.0);
+        c2l = (float)c2;
+        s2l = (float)s2;
+        bt.psi[lane] = psi;
+        bt.acc[lane][0] = 0.0;
+        bt.acc[lane][1] = 0.0;
+    }
+    float T0[G], C2P[G], S2P[G], NSF[G], MK[G], CUT_IN[G], CUT_OUT[G];
+    int NPIX[G];
+    const float *BEAM[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+        T0[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t0l), k));
+        C2P[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c2l), k));
+        S2P[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, s2l), k));
+        const BatchPair &bp_ = p.pairs[d0 + k];
+        NSF[k] = bp_.nside_f;
+        MK[k] = bp_.marg_k;
+        CUT_IN[k] = bp_.omc_cut * (1.0f - 4e-6f);
+        CUT_OUT[k] = bp_.omc_cut * (1.0f + 4e-6f);
+        NPIX[k] = bp_.npix;
+        BEAM[k] = bp_.beam;
+    }
+    Disc d;
+    disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.pairs[d0].rho_max, p.pairs[d0].cos_rmax, d);
+    wave_sync();
+
+    double acc_a[G], acc_b[G];
+#pragma unroll
+    for (int k = 0; k < G; k++)
+        acc_a[k] = acc_b[k] = 0.0;
+    int qn = 0;
+
+    for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+        const int total = build_runs(p.hs, d, r0, lane, t);
+        if (total <= 0)
+            continue;
+        int s = 0;
+        int v = lane;
+        if (v < total)
+            while (v >= t.pref[s + 1])
+                s++;
+        RingRec rr = p.rings[t.ring[s]];
+        __builtin_amdgcn_s_waitcnt(0);
+        /* pending visit: sky and, per pair, beam row and chi */
+        float qI = 0.f, qQ = 0.f, qU = 0.f;
+        float4 q0[G];
+        float2 q1[G];
+        float qc[G], qs[G];
+        unsigned qlive = 0;
+        for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
+            const bool act = v < total;
+            const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
+            const int r = t.ring[s];
+            const VisitRing g = visit_ring_f32(c, ring_phi(p.hs, rr, r, pix), rr.cdp, rr.sdp);
+            const int vn = v + kWaveSize;
+            if (vn < total)
+                while (vn >= t.pref[s + 1])
+                    s++;
+            rr = p.rings[t.ring[s]];
+            const VisitGeo vg = visit_geo_f32(p.fc, sf, g);
+
+            /* sums of the previous visit */
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+                if (qlive & (1u << k)) {
+                    const float A0 = q0[k].x, A1 = q0[k].y, A2 = q0[k].z, B0 = q0[k].w;
+                    const float B1 = q1[k].x, B2 = q1[k].y, cc = qc[k], ss = qs[k];
+                    acc_a[k] += qI * A0 + qQ * (A1 * cc - A2 * ss) + qU * (A2 * cc + A1 * ss);
+                    acc_b[k] += qI * B0 + qQ * (-B1 * cc + B2 * ss) + qU * (-B2 * cc - B1 * ss);
+                }
+            }
+            /* the only drain site of the loop: the previous visit is consumed
+             * and the next loads are not issued yet (little is live across the
+             * exact path's out-of-line call) */
+            while (qn >= kWaveSize) {
+                batch_drain<G>(p, bt, d0, qn - kWaveSize, kWaveSize, lane, c);
+                qn -= kWaveSize;
+            }
+
+            /* this visit, per pair (the queue holds < 64 entries here and
+             * grows by at most G x 64 below) */
+            qlive = 0;
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+                const bool in = act && vg.omc < CUT_IN[k];
+                const bool edge = act && !in && !(vg.omc > CUT_OUT[k]);
+                float c2 = 0.f, s2 = 0.f;
+                int bp = -1;
+                if (in)
+                    bp = beam_pixel_f32_det(vg, T0[k], C2P[k], S2P[k], NSF[k], MK[k], c2, s2);
+                const bool defer = edge || bp == kNeedExact;
+                const unsigned long long m = __ballot(defer);
+                if (m) {
+                    if (defer) {
+                        const int pos = qn + lanes_below(m);
+                        bt.qpix[pos] = pix;
+                        bt.qring[pos] = r;
+                        bt.qpair[pos] = k;
+                    }
+                    qn += __popcll(m);
+                }
+                const bool live = (unsigned)bp < (unsigned)NPIX[k];
+                qlive |= live ? (1u << k) : 0u;
+                qc[k] = c2;
+                qs[k] = s2;
+                const float *row = BEAM[k] + 8 * (long)(live ? bp : 0);
+                q0[k] = *(const float4 *)row;
+                q1[k] = *(const float2 *)(row + 4);
+            }
+            const int sp = act ? pix : 0;
+            qI = p.I[sp];
+            qQ = p.Q[sp];
+            qU = p.U[sp];
+        }
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            if (qlive & (1u << k)) {
+                const float A0 = q0[k].x, A1 = q0[k].y, A2 = q0[k].z, B0 = q0[k].w;
+                const float B1 = q1[k].x, B2 = q1[k].y, cc = qc[k], ss = qs[k];
+                acc_a[k] += qI * A0 + qQ * (A1 * cc - A2 * ss) + qU * (A2 * cc + A1 * ss);
+                acc_b[k] += qI * B0 + qQ * (-B1 * cc + B2 * ss) + qU * (-B2 * cc - B1 * ss);
+            }
+        }
+        wave_sync();
+    }
+    while (qn > 0) {
+        const int cnt = qn < kWaveSize ? qn : kWaveSize;
+        batch_drain<G>(p, bt, d0, qn - cnt, cnt, lane, c);
+        qn -= cnt;
+    }
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+        const double ta = wave_sum(acc_a[k]) + bt.acc[k][0];
+        const double tb = wave_sum(acc_b[k]) + bt.acc[k][1];
+        const int out = p.pairs[d0 + k].out;
+        if (lane == 0 && out >= 0) {
+            p.tod[(2 * (long)out) * p.nsamples + i] = (float)ta;
+            p.tod[(2 * (long)out + 1) * p.nsamples + i] = (float)tb;
+        }
+    }
+}
+
 /* ---- adjoint with respect to the sky, LDS-privatised ---------------------
  *
  * A block (16 waves) owns a chunk of K consecutive samples.  Their discs
@@ -889,6 +1144,20 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
     case OP_FWD: hipLaunchKernelGGL(k_pair<OP_FWD>, dim3(g), dim3(kBlock), 0, st, p); break;
     case OP_ADJ_MAP: hipLaunchKernelGGL(k_pair<OP_ADJ_MAP>, dim3(g), dim3(kBlock), 0, st, p); break;
     case OP_ADJ_BEAM: hipLaunchKernelGGL(k_pair<OP_ADJ_BEAM>, dim3(g), dim3(kBlock), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
+{
+    const long n = p.s_end - p.s_begin;
+    if (n <= 0)
+        return hipSuccess;
+    const unsigned g = nblocks(n * p.ngroups, kWavesPerBlock);
+    switch (group) {
+    case 2: hipLaunchKernelGGL(k_batch_fwd<2>, dim3(g), dim3(kBlock), 0, st, p); break;
+    case 4: hipLaunchKernelGGL(k_batch_fwd<4>, dim3(g), dim3(kBlock), 0, st, p); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -43,6 +43,39 @@ struct PairParams {
     double *gbeam;            /* [npix_beam][8], same row layout as beam */
 };
 
+/* One detector pair of a batched launch: its own beam (rows A0 A1 A2 B0 B1
+ * B2 A3 B3 of a map of NSIDE nside), its disc radius and the offset its
+ * position angle has from the shared pointing.  The device table is sorted by
+ * rho_max, largest first. */
+struct BatchPair {
+    const float *beam;
+    int npix;               /* beam pixels */
+    int nside;              /* beam NSIDE */
+    float nside_f, marg_k;  /* FastConsts of this beam NSIDE */
+    float omc_cut;          /* 1 - cos(rho_max) */
+    float pa_off;           /* pa of the pair = (float)(pa + pa_off) */
+    double rho_max, cos_rmax;
+    int out;                /* position of the pair in the caller's order */
+    int pad;
+};
+
+/* A batched launch over samples [s_begin, s_end) and pairs [0, npairs). */
+struct BatchParams {
+    Hpx hs;
+    const RingRec *rings;
+    const float *I, *Q, *U;
+    const BatchPair *pairs;   /* device, sorted by rho_max descending */
+    int npairs, ngroups;      /* ngroups = ceil(npairs / G) */
+    FastConsts fc;            /* rho > 1e-6 band (omc_lo, omc_hi) */
+    const float *ra, *dec, *pa;
+    long s_begin, s_end, nsamples;
+    float *tod;               /* [out][2][nsamples] */
+    const float *g;           /* cotangents, same layout */
+    double *gI, *gQ, *gU;
+};
+
+hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st);
+
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
