@@ -120,7 +120,9 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
  * per CU), "adjmap_support" (1: windows cover only the discs of the beam's
  * non-zero rows, 0: the full rho_max discs; results are the same either way),
- * "batch_group" (detector pairs per wave in batched launches: 2 or 4);
+ * "batch_group" (batched forward: 1 = one wave per (pair, sample), the
+ * single-pair computation; 2 or 4 = pairs per wave sharing each visit's
+ * geometry);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);
@@ -185,6 +187,14 @@ int pisco_dev_batched_forward(pisco_ctx *ctx, void *stream, const pisco_batch *b
                               int nside_sky, const float *I, const float *Q,
                               const float *U, long nsamples, const float *ra,
                               const float *dec, const float *pa, float *tod);
+
+/* Sky adjoint of the batched forward: gI,gQ,gU (device, 12*nside^2 floats
+ * each) = sum over pairs of A_k^T g_k, with g (device) in the layout of
+ * pisco_dev_batched_forward's tod. */
+int pisco_dev_batched_adjoint_map(pisco_ctx *ctx, void *stream, const pisco_batch *batch,
+                                  int nside_sky, long nsamples, const float *ra,
+                                  const float *dec, const float *pa, const float *g,
+                                  float *gI, float *gQ, float *gU);
 
 /* ---- HEALPix / matching primitives on the device (for verification) ---- */
 

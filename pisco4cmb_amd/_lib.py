@@ -71,6 +71,8 @@ SIGNATURES = {
     "pisco_batch_destroy": (None, [c_void_p]),
     "pisco_dev_batched_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P, P, P, c_long,
                                           P, P, P, P]),
+    "pisco_dev_batched_adjoint_map": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_long,
+                                              P, P, P, P, P, P, P]),
     "pisco_dev_pix2ang": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_ang2pix": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_get_interpol": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P, P]),

@@ -5,6 +5,8 @@ pisco_dev_batched_forward, include/pisco.h).
 
     batch = DetectorBatch([(beam_a, beam_b, nside_beam, pa_offset), ...])
     tod = batch.forward(I, Q, U, ra, dec, pa, nside_sky)   # [npairs, 2, n]
+    gI, gQ, gU = batch.adjoint_map(g, ra, dec, pa, nside_sky)
+    tod = batched_convolve(I, Q, U, batch, ra, dec, pa, nside_sky)  # autograd
 
 beam_a, beam_b: float32 [4, npix] cuda tensors (PolBeam aBeams / bBeams);
 pair k's TOD is what the single-pair forward gives with pa + pa_offset (in
@@ -55,6 +57,22 @@ class DetectorBatch:
                                             pa.data_ptr(), tod.data_ptr()))
         return tod
 
+    def adjoint_map(self, g, ra, dec, pa, nside_sky):
+        """Sky gradient of <g, tod> summed over pairs: (gI, gQ, gU) float32
+        maps; g is [npairs, 2, nsamples] like forward()'s output."""
+        g = _req(g, "g")
+        ra, dec, pa = (_req(x, n) for x, n in ((ra, "ra"), (dec, "dec"), (pa, "pa")))
+        n = ra.numel()
+        if g.numel() != self.npairs * 2 * n:
+            raise ValueError("g must be [npairs, 2, nsamples]")
+        npix = 12 * nside_sky * nside_sky
+        out = [torch.empty(npix, dtype=torch.float32, device=self.device) for _ in range(3)]
+        check(lib.pisco_dev_batched_adjoint_map(self.ctx, _stream(self.device), self.handle,
+                                                nside_sky, n, ra.data_ptr(), dec.data_ptr(),
+                                                pa.data_ptr(), g.data_ptr(),
+                                                *[o.data_ptr() for o in out]))
+        return tuple(out)
+
     def close(self):
         if getattr(self, "handle", None):
             lib.pisco_batch_destroy(self.handle)
@@ -65,3 +83,26 @@ class DetectorBatch:
             self.close()
         except Exception:
             pass
+
+
+class BatchedConvolution(torch.autograd.Function):
+    """tod = batch.forward(I, Q, U, ...), differentiable in the sky; the
+    backward runs the batched LDS adjoint (pixel lists recomputed)."""
+
+    @staticmethod
+    def forward(ctx, I, Q, U, batch, ra, dec, pa, nside_sky):
+        ctx.batch = batch
+        ctx.nside_sky = nside_sky
+        ctx.save_for_backward(ra, dec, pa)
+        return batch.forward(I, Q, U, ra, dec, pa, nside_sky)
+
+    @staticmethod
+    def backward(ctx, g):
+        ra, dec, pa = ctx.saved_tensors
+        gI, gQ, gU = ctx.batch.adjoint_map(g.contiguous().float(), ra, dec, pa, ctx.nside_sky)
+        return gI, gQ, gU, None, None, None, None, None
+
+
+def batched_convolve(I, Q, U, batch, ra, dec, pa, nside_sky):
+    """Differentiable TOD [npairs, 2, nsamples] of a DetectorBatch."""
+    return BatchedConvolution.apply(I, Q, U, batch, ra, dec, pa, nside_sky)

@@ -110,7 +110,7 @@ struct pisco_ctx {
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
-    int batch_group = 2;          /* detector pairs per wave in batched launches */
+    int batch_group = 1;          /* pairs per wave in batched forwards (1: per-pair items) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
 };
 
@@ -278,8 +278,8 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
             return fail(PISCO_ERR_INVALID, "adjmap_support must be 0 or 1");
         c->adjmap_support = (int)value;
     } else if (k == "batch_group") {
-        if (value != 2 && value != 4)
-            return fail(PISCO_ERR_INVALID, "batch_group must be 2 or 4");
+        if (value != 1 && value != 2 && value != 4)
+            return fail(PISCO_ERR_INVALID, "batch_group must be 1, 2 or 4");
         c->batch_group = (int)value;
     } else if (k == "adjmap_blocks") {
         if (value < 0 || value > 65536)
@@ -860,6 +860,7 @@ struct pisco_batch {
     int npad = 0;                 /* table length: npairs rounded up to 4 */
     BatchPair *table = nullptr;   /* device, sorted by rho_max descending */
     std::vector<BatchPair> host;  /* same, on the host */
+    double rho_support = 0.0;     /* union over pairs of the non-zero beam support */
 };
 
 int pisco_batch_create(pisco_ctx *c, int npairs, const pisco_batch_pair *pairs,
@@ -911,12 +912,42 @@ int pisco_batch_create(pisco_ctx *c, int npairs, const pisco_batch_pair *pairs,
         (void)hipFree(d);
         return fail(PISCO_ERR_DEVICE, "hipMemcpy failed: %s", hipGetErrorString(e));
     }
+    /* union of the pairs' non-zero beam supports (sizes the sky adjoint's
+     * LDS windows; see k_adjmap_lds): theta(ring of the last non-zero row + 2)
+     * capped at the pair's rho_max */
+    double support = 0.0;
+    {
+        int *last = nullptr;
+        e = hipMalloc((void **)&last, sizeof(int) * (size_t)npairs);
+        std::vector<int> hl((size_t)npairs, -1);
+        for (int k = 0; k < npairs && e == hipSuccess; k++)
+            e = launch_beam_support(pairs[k].beam8, pairs[k].npix_beam, last + k, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hl.data(), last, sizeof(int) * (size_t)npairs,
+                               hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (last)
+            (void)hipFree(last);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return fail(PISCO_ERR_DEVICE, "beam support: %s", hipGetErrorString(e));
+        }
+        for (int k = 0; k < npairs; k++) {
+            const Hpx hb = hpx_make(pairs[k].nside_beam);
+            const double rmax = pisco_beam_rho_max(pairs[k].nside_beam, pairs[k].npix_beam);
+            const int rq = (hl[(size_t)k] < 0 ? 0 : pix2ring(hb, hl[(size_t)k])) + 2;
+            double th = rq < 4 * hb.nside ? ring_theta(hb, rq) : rmax;
+            support = std::max(support, std::min(th, rmax));
+        }
+    }
     pisco_batch *bt = new pisco_batch();
     bt->device = c->device;
     bt->npairs = npairs;
     bt->npad = npad;
     bt->table = d;
     bt->host = std::move(t);
+    bt->rho_support = support;
     *out = bt;
     return PISCO_OK;
 }
@@ -930,6 +961,36 @@ void pisco_batch_destroy(pisco_batch *b)
     delete b;
 }
 
+namespace {
+
+int setup_batch(pisco_ctx *c, const pisco_batch *b, int nside_sky, const float *I,
+                const float *Q, const float *U, long nsamples, const float *ra,
+                const float *dec, const float *pa, BatchParams &p)
+{
+    const RingRec *rt = nullptr;
+    CHK(ring_table(c, nside_sky, &rt));
+    memset(&p, 0, sizeof p);
+    p.hs = hpx_make(nside_sky);
+    p.rings = rt;
+    p.I = I;
+    p.Q = Q;
+    p.U = U;
+    p.pairs = b->table;
+    p.npairs = b->npad;
+    p.ngroups = b->npad;
+    p.fc = fast_consts(hpx_make(1));     /* only the rho > 1e-6 band is read */
+    p.fc64 = fast_consts64(hpx_make(1)); /* only the crho thresholds are read */
+    p.ra = ra;
+    p.dec = dec;
+    p.pa = pa;
+    p.nsamples = nsamples;
+    p.s_begin = 0;
+    p.s_end = nsamples;
+    return PISCO_OK;
+}
+
+} // namespace
+
 int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, int nside_sky,
                               const float *I, const float *Q, const float *U, long nsamples,
                               const float *ra, const float *dec, const float *pa, float *tod)
@@ -941,24 +1002,10 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
     if (b->device != c->device)
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
     HIPCHK(hipSetDevice(c->device));
-    const RingRec *rt = nullptr;
-    CHK(ring_table(c, nside_sky, &rt));
     BatchParams p;
-    memset(&p, 0, sizeof p);
-    p.hs = hpx_make(nside_sky);
-    p.rings = rt;
-    p.I = I;
-    p.Q = Q;
-    p.U = U;
-    p.pairs = b->table;
-    p.npairs = b->npad;
+    CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
     const int G = c->batch_group;
     p.ngroups = b->npad / G;
-    p.fc = fast_consts(hpx_make(1));   /* only the rho > 1e-6 band is read */
-    p.ra = ra;
-    p.dec = dec;
-    p.pa = pa;
-    p.nsamples = nsamples;
     p.tod = tod;
     hipStream_t st = pick(c, stream);
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : nsamples;
@@ -968,5 +1015,58 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
         p.s_end = std::min(nsamples, s0 + chunk);
         HIPCHK(launch_batch_forward(G, p, st));
     }
+    return PISCO_OK;
+}
+
+int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch *b,
+                                  int nside_sky, long nsamples, const float *ra,
+                                  const float *dec, const float *pa, const float *g, float *gI,
+                                  float *gQ, float *gU)
+{
+    if (!c || !b || !g || !gI || !gQ || !gU || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (!valid_nside(nside_sky) || nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (b->device != c->device)
+        return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = pick(c, stream);
+    BatchParams bp;
+    CHK(setup_batch(c, b, nside_sky, nullptr, nullptr, nullptr, nsamples, ra, dec, pa, bp));
+    const long npix = 12L * nside_sky * nside_sky;
+    CHK(c->acc.ensure(3 * (size_t)npix));
+    HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
+    bp.g = g;
+    bp.gI = c->acc.p;
+    bp.gQ = c->acc.p + npix;
+    bp.gU = c->acc.p + 2 * npix;
+    /* the union disc (the first, largest pair) drives the window passes */
+    const BatchPair &q0 = b->host[0];
+    PairParams p;
+    memset(&p, 0, sizeof p);
+    p.hs = bp.hs;
+    p.rings = bp.rings;
+    p.hb = hpx_make(q0.nside);
+    p.npix_beam = q0.npix;
+    p.rho_max = q0.rho_max;
+    p.cos_rmax = q0.cos_rmax;
+    p.ra = ra;
+    p.dec = dec;
+    p.pa = pa;
+    p.gI = bp.gI;
+    p.gQ = bp.gQ;
+    p.gU = bp.gU;
+    p.s_begin = 0;
+    p.s_end = nsamples;
+    const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+    const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
+    hipError_t e = launch_adjmap_lds(p, chunk, c->adjmap_cap, c->adjmap_rw, nb, st, &bp,
+                                     std::min(b->rho_support, q0.rho_max));
+    if (e != hipSuccess)
+        return fail(PISCO_ERR_DEVICE, "batched sky adjoint launch failed: %s",
+                    hipGetErrorString(e));
+    HIPCHK(launch_f64_to_f32(bp.gI, gI, npix, st));
+    HIPCHK(launch_f64_to_f32(bp.gQ, gQ, npix, st));
+    HIPCHK(launch_f64_to_f32(bp.gU, gU, npix, st));
     return PISCO_OK;
 }

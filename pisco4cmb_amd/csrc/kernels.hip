@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 
 #include "device_hpx.h"
 #include "kernels.hpp"
@@ -313,10 +314,10 @@ __device__ inline float unif(float x)
 
 /* Per-sample setup shared by the visit kernels: beam-centre constants in
  * scalar registers and the sample's disc. */
-__device__ inline void sample_setup(const PairParams &p, long i, Centre &c, SampleF &sf,
-                                    SampleD &sd, Disc &d)
+__device__ inline void sample_setup(const PairParams &p, long i, float pa_i, Centre &c,
+                                    SampleF &sf, SampleD &sd, Disc &d)
 {
-    c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+    c = centre_make(p.ra[i], p.dec[i], pa_i);
     c.ra = uni(c.ra);
     c.dec = uni(c.dec);
     c.psi = uni(c.psi);
@@ -335,24 +336,17 @@ __device__ inline void sample_setup(const PairParams &p, long i, Centre &c, Samp
     disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
 }
 
+/* One sample of one detector pair (pa of the sample given): forward TOD,
+ * or the direct-atomic adjoints. */
 template <int OP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pair(PairParams p)
+__device__ inline void pair_sample(const PairParams &p, long i, float pa_i, RunTable &t,
+                                   int lane)
 {
-    __shared__ RunTable tables[kWavesPerBlock];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
-    const int lane = threadIdx.x % kWaveSize;
-    RunTable &t = tables[wave];
-
-    long lb = xcd_remap(blockIdx.x, gridDim.x);
-    long i = p.s_begin + lb * kWavesPerBlock + wave;
-    if (i >= p.s_end)
-        return;
-
     Centre c;
     SampleF sf;
     SampleD sd;
     Disc d;
-    sample_setup(p, i, c, sf, sd, d);
+    sample_setup(p, i, pa_i, c, sf, sd, d);
     double ga = 0.0, gb = 0.0;
     if (OP != OP_FWD) {
         ga = uni(p.g_a ? (double)p.g_a[i] : 0.0);
@@ -411,11 +405,88 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     }
 }
 
+template <int OP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pair(PairParams p)
+{
+    __shared__ RunTable tables[kWavesPerBlock];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
+    const int lane = threadIdx.x % kWaveSize;
+    long lb = xcd_remap(blockIdx.x, gridDim.x);
+    long i = p.s_begin + lb * kWavesPerBlock + wave;
+    if (i >= p.s_end)
+        return;
+    pair_sample<OP>(p, i, p.pa[i], tables[wave], lane);
+}
+
+/* PairParams of one pair of a batch: its beam, beam-map geometry, disc and
+ * outputs (TOD rows 2 out, 2 out + 1 of [pairs][2][nsamples]). */
+__device__ inline PairParams batch_pair_params(const BatchParams &b, const BatchPair &q)
+{
+    PairParams p;
+    p.hs = b.hs;
+    p.rings = b.rings;
+    p.I = b.I;
+    p.Q = b.Q;
+    p.U = b.U;
+    p.hb = hpx_make(q.nside);
+    p.npix_beam = q.npix;
+    p.rho_max = q.rho_max;
+    p.cos_rmax = q.cos_rmax;
+    p.fc.omc_lo = b.fc.omc_lo;
+    p.fc.omc_hi = b.fc.omc_hi;
+    p.fc.marg_k = q.marg_k;
+    p.fc.nside = q.nside_f;
+    p.fc64.crho_big = b.fc64.crho_big;
+    p.fc64.crho_small = b.fc64.crho_small;
+    p.fc64.marg_k = 100.0 * 4.4e-16 * q.nside * 1.7320508075688772;
+    p.beam = q.beam;
+    p.beam_last = nullptr;
+    p.ra = b.ra;
+    p.dec = b.dec;
+    p.pa = b.pa;
+    p.s_begin = b.s_begin;
+    p.s_end = b.s_end;
+    const long row = 2 * (long)(q.out < 0 ? 0 : q.out) * b.nsamples;
+    p.tod_a = b.tod ? b.tod + row : nullptr;
+    p.tod_b = b.tod ? b.tod + row + b.nsamples : nullptr;
+    p.g_a = b.g ? b.g + row : nullptr;
+    p.g_b = b.g ? b.g + row + b.nsamples : nullptr;
+    p.gI = b.gI;
+    p.gQ = b.gQ;
+    p.gU = b.gU;
+    p.gbeam = nullptr;
+    return p;
+}
+
+/* Batched form of k_pair: one wave per (pair, sample) item, pairs outermost
+ * (concurrent waves share one beam, which stays in L2); each item is exactly
+ * the single-pair computation with pa = (float)(pa + pa_offset). */
+template <int OP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_batch_pair(BatchParams b)
+{
+    __shared__ RunTable tables[kWavesPerBlock];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
+    const int lane = threadIdx.x % kWaveSize;
+    const long lb = xcd_remap(blockIdx.x, gridDim.x);
+    const long w = lb * kWavesPerBlock + wave;
+    const long nsamp = b.s_end - b.s_begin;
+    if (w >= nsamp * b.npairs)
+        return;
+    const BatchPair &q = b.pairs[w / nsamp];
+    if (q.out < 0)
+        return;   /* inert padding */
+    const long i = b.s_begin + w % nsamp;
+    const PairParams p = batch_pair_params(b, q);
+    pair_sample<OP>(p, i, b.pa[i] + q.pa_off, tables[wave], lane);
+}
+
 /* ---- batched forward: many detector pairs on one pointing ----------------
  *
  * Detector pairs that share the boresight pointing (each with its own beam,
  * disc radius and position-angle offset) are convolved in one launch.  A wave
- * takes one (sample, group of G pairs); the pairs are sorted by disc radius,
+ * takes one (sample, group of G pairs), groups outermost; the pairs are sorted
+ * by disc radius,
  * so the group's union disc is the disc of its first pair.  Lanes walk the
  * visits of that disc as in k_pair; per visit the parts that do not depend on
  * the pair (sky values, the float geometry: 1 - cos rho, alpha, the chi
@@ -504,8 +575,10 @@ void k_batch_fwd(BatchParams p)
     const long nsamp = p.s_end - p.s_begin;
     if (w >= nsamp * p.ngroups)
         return;
-    const long i = p.s_begin + w / p.ngroups;
-    const int d0 = (int)(w % p.ngroups) * G;
+    /* group-major: concurrent waves work on the same few beams, which then
+     * stay in L2 (sample-major order would cycle through every beam) */
+    const long i = p.s_begin + w % nsamp;
+    const int d0 = (int)(w / nsamp) * G;
 
     Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
     c.ra = uni(c.ra);
@@ -683,6 +756,12 @@ struct AdjLayout {
     int cap;         /* LDS pixel capacity of the window */
     int rw;          /* ring capacity of the window tables */
     long nchunks;
+    /* batch mode (pairs != NULL): every pair of b.pairs adds into the same
+     * windows; the windows use radius rho_w (the pairs' union support) */
+    const BatchPair *pairs;
+    int npairs;
+    double rho_w, cos_w;
+    BatchParams b;
 };
 
 #ifndef PISCO_ADJ_WAVES
@@ -703,6 +782,7 @@ __device__ inline void adj_visit_values(float ga, float gb, float4 v0, float2 v1
     vU = ga * (A2 * c2 + A1 * s2) + gb * (-B2 * c2 - B1 * s2);
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -727,7 +807,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
      * take the global-atomic path below, so the windows only need to be
      * right for speed, not for the result. */
     double rho_w = p.rho_max, cos_w = p.cos_rmax;
-    if (p.beam_last) {
+    if (BATCH) {
+        rho_w = L.rho_w;
+        cos_w = L.cos_w;
+    } else if (p.beam_last) {
         int last = __builtin_amdgcn_readfirstlane(*p.beam_last);
         int rq = (last < 0 ? 0 : pix2ring(p.hb, last)) + 2;
         if (rq < 4 * p.hb.nside) {
@@ -850,17 +933,31 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             __syncthreads();
         }
 
-        /* pass 2: visits, one wave per sample */
+        /* pass 2: visits, one wave per (pair, sample), pairs outermost */
         const int W = fallback ? 0 : ctl[2];
-        for (int j = wave; j < ns; j += kAdjWaves) {
-            long i = s0 + j;
+        const int np = BATCH ? L.npairs : 1;
+        for (int j = wave; j < ns * np; j += kAdjWaves) {
+            const int jp = j / ns;
+            const long i = s0 + (j - jp * ns);
+            float pa_i;
+            PairParams pp;
+            if constexpr (BATCH) {
+                const BatchPair &q = L.pairs[jp];
+                if (q.out < 0)
+                    continue;   /* inert padding (uniform per wave) */
+                pp = batch_pair_params(L.b, q);
+                pa_i = p.pa[i] + q.pa_off;
+            } else {
+                pp = p;
+                pa_i = p.pa[i];
+            }
             Centre c;
             SampleF sf;
             SampleD sd;
             Disc d;
-            sample_setup(p, i, c, sf, sd, d);
-            const float ga = unif(p.g_a ? p.g_a[i] : 0.f);
-            const float gb = unif(p.g_b ? p.g_b[i] : 0.f);
+            sample_setup(pp, i, pa_i, c, sf, sd, d);
+            const float ga = unif(pp.g_a ? pp.g_a[i] : 0.f);
+            const float gb = unif(pp.g_b ? pp.g_b[i] : 0.f);
             auto use = [&](const Pending &q) {
                 const int pix = q.pix, r = q.r;
                 float vI, vQ, vU;
@@ -870,8 +967,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 const int k = r - R0;
                 int slot = -1;
                 if (!fallback && (unsigned)k < (unsigned)nring) {
-                    int nr = 4 * (r < p.hs.nside ? r
-                                  : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r : p.hs.nside));
+                    int nr = 4 * (r < pp.hs.nside ? r
+                                  : (r > 3 * pp.hs.nside ? 4 * pp.hs.nside - r : pp.hs.nside));
                     int off = pix - q.sp - win_lo[k];
                     if (off < 0)
                         off += nr;
@@ -883,19 +980,19 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                     atomicAdd(&acc[W + slot], vQ);
                     atomicAdd(&acc[2 * W + slot], vU);
                 } else {   /* no window holds the pixel: global FP64 atomics */
-                    unsafeAtomicAdd(&p.gI[pix], (double)vI);
-                    unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
-                    unsafeAtomicAdd(&p.gU[pix], (double)vU);
+                    unsafeAtomicAdd(&pp.gI[pix], (double)vI);
+                    unsafeAtomicAdd(&pp.gQ[pix], (double)vQ);
+                    unsafeAtomicAdd(&pp.gU[pix], (double)vU);
                 }
             };
             int qn = 0;
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
-                int total = build_runs(p.hs, d, r0, lane, t);
-                walk_visits<false, false>(p, t, total, lane, c, sf, sd, qn, use);
+                int total = build_runs(pp.hs, d, r0, lane, t);
+                walk_visits<false, false>(pp, t, total, lane, c, sf, sd, qn, use);
                 wave_sync();
             }
             if (qn > 0)
-                drain_exact<false>(p, t, 0, qn, lane, c, use);
+                drain_exact<false>(pp, t, 0, qn, lane, c, use);
         }
         __syncthreads();
 
@@ -1156,6 +1253,10 @@ hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
         return hipSuccess;
     const unsigned g = nblocks(n * p.ngroups, kWavesPerBlock);
     switch (group) {
+    case 1:
+        hipLaunchKernelGGL(k_batch_pair<OP_FWD>, dim3(nblocks(n * p.npairs, kWavesPerBlock)),
+                           dim3(kBlock), 0, st, p);
+        break;
     case 2: hipLaunchKernelGGL(k_batch_fwd<2>, dim3(g), dim3(kBlock), 0, st, p); break;
     case 4: hipLaunchKernelGGL(k_batch_fwd<4>, dim3(g), dim3(kBlock), 0, st, p); break;
     default: return hipErrorInvalidValue;
@@ -1170,35 +1271,46 @@ size_t adjmap_lds_bytes(int cap, int rw)
 }
 
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
-                             hipStream_t st)
+                             hipStream_t st, const BatchParams *batch, double rho_w)
 {
     long n = p.s_end - p.s_begin;
     if (n <= 0)
         return hipSuccess;
     AdjLayout L;
+    memset(&L, 0, sizeof L);
     L.chunk = chunk;
     L.cap = cap;
     L.rw = rw;
     L.nchunks = (n + chunk - 1) / chunk;
+    if (batch) {
+        L.pairs = batch->pairs;
+        L.npairs = batch->npairs;
+        L.rho_w = rho_w;
+        L.cos_w = pl_cos(rho_w);
+        L.b = *batch;
+    }
     size_t bytes = adjmap_lds_bytes(cap, rw);
-    static size_t configured = 0;
-    if (bytes > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)bytes);
+    static size_t configured[2] = {0, 0};
+    const void *fn = batch ? (const void *)k_adjmap_lds<true> : (const void *)k_adjmap_lds<false>;
+    if (bytes > configured[batch ? 1 : 0]) {
+        hipError_t e =
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess)
             return e;
-        configured = bytes;
+        configured[batch ? 1 : 0] = bytes;
     }
     long g = std::min<long>(L.nchunks, nblocks);
-    hipLaunchKernelGGL(k_adjmap_lds, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
+    if (batch)
+        hipLaunchKernelGGL(k_adjmap_lds<true>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
+    else
+        hipLaunchKernelGGL(k_adjmap_lds<false>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
     return hipGetLastError();
 }
 
 int adjmap_describe(char *buf, size_t n, int cap, int rw)
 {
     hipFuncAttributes a;
-    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds);
+    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds<false>);
     if (e != hipSuccess)
         return snprintf(buf, n, "hipFuncGetAttributes: %s", hipGetErrorString(e));
     return snprintf(buf, n,

@@ -67,6 +67,7 @@ struct BatchParams {
     const BatchPair *pairs;   /* device, sorted by rho_max descending */
     int npairs, ngroups;      /* ngroups = ceil(npairs / G) */
     FastConsts fc;            /* rho > 1e-6 band (omc_lo, omc_hi) */
+    FastConsts64 fc64;        /* crho thresholds of the double-precision form */
     const float *ra, *dec, *pa;
     long s_begin, s_end, nsamples;
     float *tod;               /* [out][2][nsamples] */
@@ -80,7 +81,8 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
-                             hipStream_t st);
+                             hipStream_t st, const BatchParams *batch = nullptr,
+                             double rho_w = 0.0);
 size_t adjmap_lds_bytes(int cap, int rw);
 /* Launch resources of the sky-adjoint kernel, for error messages. */
 int adjmap_describe(char *buf, size_t n, int cap, int rw);
