@@ -110,7 +110,7 @@ struct pisco_ctx {
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
-    int batch_group = 1;          /* pairs per wave in batched forwards (1: per-pair items) */
+    int batch_group = 4;          /* pairs per wave in batched forwards (1: per-pair items) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
 };
 

@@ -418,6 +418,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     pair_sample<OP>(p, i, p.pa[i], tables[wave], lane);
 }
 
+/* Item order of the batched launches: samples in chunks of kBatchChunk, and
+ * within a chunk all items of one pair (or pair group) before the next.  The
+ * XCD-aware block order hands each XCD whole chunks, so every XCD sees every
+ * pair (the pairs' discs differ widely in size) while consecutive waves
+ * share one beam.  Returns false for the padding items of a partial last
+ * chunk. */
+constexpr long kBatchChunk = 1024;
+
+__device__ inline bool batch_item(long w, long nsamp, int nunits, long &unit, long &sample)
+{
+    const long per = kBatchChunk * nunits;
+    const long c = w / per;
+    const long rem = w - c * per;
+    const long cs = c * kBatchChunk;
+    const long len = (nsamp - cs) < kBatchChunk ? (nsamp - cs) : kBatchChunk;
+    if (len <= 0 || rem >= len * nunits)
+        return false;
+    unit = rem / len;
+    sample = cs + (rem - unit * len);
+    return true;
+}
+
+__host__ inline long batch_items(long nsamp, int nunits)
+{
+    return (nsamp + kBatchChunk - 1) / kBatchChunk * kBatchChunk * nunits;
+}
+
 /* PairParams of one pair of a batch: its beam, beam-map geometry, disc and
  * outputs (TOD rows 2 out, 2 out + 1 of [pairs][2][nsamples]). */
 __device__ inline PairParams batch_pair_params(const BatchParams &b, const BatchPair &q)
@@ -458,9 +485,9 @@ __device__ inline PairParams batch_pair_params(const BatchParams &b, const Batch
     return p;
 }
 
-/* Batched form of k_pair: one wave per (pair, sample) item, pairs outermost
- * (concurrent waves share one beam, which stays in L2); each item is exactly
- * the single-pair computation with pa = (float)(pa + pa_offset). */
+/* Batched form of k_pair: one wave per (pair, sample) item in the order of
+ * batch_item; each item is exactly the single-pair computation with
+ * pa = (float)(pa + pa_offset). */
 template <int OP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void k_batch_pair(BatchParams b)
@@ -470,13 +497,13 @@ void k_batch_pair(BatchParams b)
     const int lane = threadIdx.x % kWaveSize;
     const long lb = xcd_remap(blockIdx.x, gridDim.x);
     const long w = lb * kWavesPerBlock + wave;
-    const long nsamp = b.s_end - b.s_begin;
-    if (w >= nsamp * b.npairs)
+    long k, j;
+    if (!batch_item(w, b.s_end - b.s_begin, b.npairs, k, j))
         return;
-    const BatchPair &q = b.pairs[w / nsamp];
+    const BatchPair &q = b.pairs[k];
     if (q.out < 0)
         return;   /* inert padding */
-    const long i = b.s_begin + w % nsamp;
+    const long i = b.s_begin + j;
     const PairParams p = batch_pair_params(b, q);
     pair_sample<OP>(p, i, b.pa[i] + q.pa_off, tables[wave], lane);
 }
@@ -485,8 +512,8 @@ void k_batch_pair(BatchParams b)
  *
  * Detector pairs that share the boresight pointing (each with its own beam,
  * disc radius and position-angle offset) are convolved in one launch.  A wave
- * takes one (sample, group of G pairs), groups outermost; the pairs are sorted
- * by disc radius,
+ * takes one (sample, group of G pairs), in the item order of batch_item; the
+ * pairs are sorted by disc radius,
  * so the group's union disc is the disc of its first pair.  Lanes walk the
  * visits of that disc as in k_pair; per visit the parts that do not depend on
  * the pair (sky values, the float geometry: 1 - cos rho, alpha, the chi
@@ -572,13 +599,11 @@ void k_batch_fwd(BatchParams p)
 
     const long lb = xcd_remap(blockIdx.x, gridDim.x);
     const long w = lb * kWavesPerBlock + wave;
-    const long nsamp = p.s_end - p.s_begin;
-    if (w >= nsamp * p.ngroups)
+    long grp, j;
+    if (!batch_item(w, p.s_end - p.s_begin, p.ngroups, grp, j))
         return;
-    /* group-major: concurrent waves work on the same few beams, which then
-     * stay in L2 (sample-major order would cycle through every beam) */
-    const long i = p.s_begin + w % nsamp;
-    const int d0 = (int)(w / nsamp) * G;
+    const long i = p.s_begin + j;
+    const int d0 = (int)grp * G;
 
     Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
     c.ra = uni(c.ra);
@@ -1251,10 +1276,11 @@ hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
     const long n = p.s_end - p.s_begin;
     if (n <= 0)
         return hipSuccess;
-    const unsigned g = nblocks(n * p.ngroups, kWavesPerBlock);
+    const unsigned g = nblocks(batch_items(n, p.ngroups), kWavesPerBlock);
     switch (group) {
     case 1:
-        hipLaunchKernelGGL(k_batch_pair<OP_FWD>, dim3(nblocks(n * p.npairs, kWavesPerBlock)),
+        hipLaunchKernelGGL(k_batch_pair<OP_FWD>,
+                           dim3(nblocks(batch_items(n, p.npairs), kWavesPerBlock)),
                            dim3(kBlock), 0, st, p);
         break;
     case 2: hipLaunchKernelGGL(k_batch_fwd<2>, dim3(g), dim3(kBlock), 0, st, p); break;

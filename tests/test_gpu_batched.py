@@ -63,7 +63,7 @@ def test_batched_forward_matches_single_pair_and_oracle(case, oracle, group):
             ra_, rb_ = oracle.convolve(NS, *sky[:3], nb, list(b.aBeams) + list(b.bBeams),
                                        ra, dec, pa_k)
             assert rel_rms(tod[k, 0], ra_) < 1e-6 and rel_rms(tod[k, 1], rb_) < 1e-6
-    check(lib.pisco_set_tuning(ctx, b"batch_group", 2))
+    check(lib.pisco_set_tuning(ctx, b"batch_group", 4))
     batch.close()
 
 
