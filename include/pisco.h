@@ -122,7 +122,8 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * non-zero rows, 0: the full rho_max discs; results are the same either way),
  * "batch_group" (batched forward: 1 = one wave per (pair, sample), the
  * single-pair computation; 2 or 4 = pairs per wave sharing each visit's
- * geometry);
+ * geometry), "batch_adj_group" (batched sky adjoint: 1, 2 or 4 pairs per
+ * wave, as batch_group);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);

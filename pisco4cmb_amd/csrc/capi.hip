@@ -111,6 +111,7 @@ struct pisco_ctx {
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
     int batch_group = 4;          /* pairs per wave in batched forwards (1: per-pair items) */
+    int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
     DevBuf<int> support;          /* last non-zero beam row (device) */
 };
 
@@ -281,6 +282,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 1 && value != 2 && value != 4)
             return fail(PISCO_ERR_INVALID, "batch_group must be 1, 2 or 4");
         c->batch_group = (int)value;
+    } else if (k == "batch_adj_group") {
+        if (value != 1 && value != 2 && value != 4)
+            return fail(PISCO_ERR_INVALID, "batch_adj_group must be 1, 2 or 4");
+        c->batch_adj_group = (int)value;
     } else if (k == "adjmap_blocks") {
         if (value < 0 || value > 65536)
             return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
@@ -308,6 +313,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjmap_support;
     else if (k == "batch_group")
         *value = c->batch_group;
+    else if (k == "batch_adj_group")
+        *value = c->batch_adj_group;
     else if (k == "cu_count")
         *value = c->ncu;
     else
@@ -1060,8 +1067,14 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     p.s_end = nsamples;
     const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
     const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
-    hipError_t e = launch_adjmap_lds(p, chunk, c->adjmap_cap, c->adjmap_rw, nb, st, &bp,
-                                     std::min(b->rho_support, q0.rho_max));
+    const int grp = c->batch_adj_group;
+    bp.ngroups = b->npad / grp;
+    /* the window capacity that fits next to the group's per-wave tables */
+    int cap = c->adjmap_cap;
+    while (cap > 256 && adjmap_lds_bytes(cap, c->adjmap_rw, grp) > 163840)
+        cap -= 256;
+    hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, &bp,
+                                     std::min(b->rho_support, q0.rho_max), grp);
     if (e != hipSuccess)
         return fail(PISCO_ERR_DEVICE, "batched sky adjoint launch failed: %s",
                     hipGetErrorString(e));

@@ -525,23 +525,25 @@ void k_batch_pair(BatchParams b)
  * evaluated exactly, their sums kept in LDS.  The pair table is padded to a
  * multiple of G with inert pairs (npix 0, never members). */
 
-constexpr int kMaxGroup = 4;
-constexpr int kBatchQueue = (kMaxGroup + 1) * kWaveSize;   /* one visit adds <= G x 64 */
-
+/* Per-wave LDS of a group of G pairs; the queue holds < 64 entries between
+ * drains and one visit adds at most G x 64. */
+template <int G>
 struct BatchTable {
-    RunTable rt;                    /* run table (its own queue is not used) */
-    int qpix[kBatchQueue];          /* queue: sky pixel, ring, pair in the group */
-    int qring[kBatchQueue];
-    int qpair[kBatchQueue];
-    double acc[kMaxGroup][2];       /* exact-path sums of each pair */
-    double psi[kMaxGroup];          /* psi of each pair at this sample */
+    RunTable rt;                       /* run table (its own queue is not used) */
+    int qpix[(G + 1) * kWaveSize];     /* queue: sky pixel, ring, pair in the group */
+    int qring[(G + 1) * kWaveSize];
+    int qpair[(G + 1) * kWaveSize];
+    double acc[G][2];                  /* forward: exact-path sums of each pair */
+    double psi[G];                     /* psi of each pair at this sample */
+    float ga[G], gb[G];                /* adjoint: cotangents of each pair */
 };
 
-/* Exact contribution (forward) of a queued visit to pair d: membership by
- * the query_disc ring test at the pair's radius, then the exact mapping. */
+/* Exact mapping of a queued visit for one pair: membership by the query_disc
+ * ring test at the pair's radius, then the exact beam pixel and chi; the
+ * visit's beam row (and its ring's first pixel) when it is inside the beam. */
 __device__ inline bool batch_exact_visit(const BatchParams &p, const BatchPair &bp_,
                                          const Centre &c, double psi, int pix, int r,
-                                         float &ta, float &tb)
+                                         ExactVisit &e, float4 &v0, float2 &v1, int &sp)
 {
     Disc dk;
     disc_setup(p.hs, kHalfPi - c.dec, c.ra, bp_.rho_max, bp_.cos_rmax, dk);
@@ -556,46 +558,206 @@ __device__ inline bool batch_exact_visit(const BatchParams &p, const BatchPair &
         return false;
     const RingRec rr = p.rings[r];
     const double ra_pix = ring_phi(p.hs, rr, r, pix);
-    const ExactVisit e =
-        beam_visit_exact(bp_.nside, c.ra, psi, c.cdc, c.sdc, ra_pix, rr.cdp, rr.sdp);
+    e = beam_visit_exact(bp_.nside, c.ra, psi, c.cdc, c.sdc, ra_pix, rr.cdp, rr.sdp);
     if ((unsigned)e.bp >= (unsigned)bp_.npix)
         return false;
     const float *row = bp_.beam + 8 * (long)e.bp;
-    const float4 v0 = *(const float4 *)row;
-    const float2 v1 = *(const float2 *)(row + 4);
-    const float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
-    ta = sI * v0.x + sQ * (v0.y * e.c2 - v0.z * e.s2) + sU * (v0.z * e.c2 + v0.y * e.s2);
-    tb = sI * v0.w + sQ * (-v1.x * e.c2 + v1.y * e.s2) + sU * (-v1.y * e.c2 - v1.x * e.s2);
+    v0 = *(const float4 *)row;
+    v1 = *(const float2 *)(row + 4);
+    sp = rr.startpix;
     return true;
 }
 
+/* Wave-uniform constants of the pairs d0 .. d0+G-1 at one sample. */
 template <int G>
-__device__ inline void batch_drain(const BatchParams &p, BatchTable &bt, int d0, int base,
-                                   int cnt, int lane, const Centre &c)
+struct BatchConsts {
+    float T0[G], C2P[G], S2P[G];   /* (pi - psi) 2/pi mod 4, cos/sin 2 psi */
+    float GA[G], GB[G];            /* adjoint cotangents */
+    float NSF[G], MK[G], CUT_IN[G], CUT_OUT[G];
+    int NPIX[G];
+    const float *BEAM[G];
+};
+
+template <int G, bool ADJ>
+__device__ inline void batch_consts(const BatchParams &p, BatchTable<G> &bt, long i, int d0,
+                                    int lane, BatchConsts<G> &K)
 {
-    wave_sync();
-    if (lane < cnt) {
-        const int pix = bt.qpix[base + lane], r = bt.qring[base + lane];
-        const int k = bt.qpair[base + lane];
-        float ta, tb;
-        if (batch_exact_visit(p, p.pairs[d0 + k], c, bt.psi[k], pix, r, ta, tb)) {
-            atomicAdd(&bt.acc[k][0], (double)ta);
-            atomicAdd(&bt.acc[k][1], (double)tb);
+    float t0l = 0.f, c2l = 1.f, s2l = 0.f, gal = 0.f, gbl = 0.f;
+    if (lane < G) {   /* lane k: pair d0 + k */
+        const BatchPair &q = p.pairs[d0 + lane];
+        const float pa_k = p.pa[i] + q.pa_off;   /* the pair's own pa */
+        const double psi = -(double)pa_k;
+        double s2, c2;
+        pl_sincos(2.0 * psi, &s2, &c2);
+        t0l = (float)fmodulo((kPi - psi) * kInvHalfPi, 4.0);
+        c2l = (float)c2;
+        s2l = (float)s2;
+        if (ADJ && q.out >= 0) {
+            const long row = 2 * (long)q.out * p.nsamples;
+            gal = p.g[row + i];
+            gbl = p.g[row + p.nsamples + i];
         }
+        bt.psi[lane] = psi;
+        bt.acc[lane][0] = 0.0;
+        bt.acc[lane][1] = 0.0;
+        bt.ga[lane] = gal;
+        bt.gb[lane] = gbl;
     }
-    __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+        K.T0[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t0l), k));
+        K.C2P[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c2l), k));
+        K.S2P[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, s2l), k));
+        K.GA[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, gal), k));
+        K.GB[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, gbl), k));
+        const BatchPair &q = p.pairs[d0 + k];
+        K.NSF[k] = q.nside_f;
+        K.MK[k] = q.marg_k;
+        K.CUT_IN[k] = q.omc_cut * (1.0f - 4e-6f);
+        K.CUT_OUT[k] = q.omc_cut * (1.0f + 4e-6f);
+        K.NPIX[k] = q.npix;
+        K.BEAM[k] = q.beam;
+    }
     wave_sync();
+}
+
+/* A visit of the group waiting for its loads: per pair the beam row and
+ * chi, and the visit's sky values / pixel. */
+template <int G>
+struct BatchPend {
+    float4 q0[G];
+    float2 q1[G];
+    float qc[G], qs[G];
+    float sI, sQ, sU;
+    int pix, r, sp;
+    unsigned live;   /* bit k: pair k has this visit inside its beam */
+};
+
+/* The walk over the visits of the group's union disc (the disc of its first
+ * pair), pipelined as walk_visits: per visit the shared float geometry, then
+ * each pair's tail (membership, beam pixel, gather); consume(q) receives a
+ * visit once its loads are due (after the next visit's geometry).  Queued
+ * visits (uncertain membership or beam pixel) are drained 64 at a time by the
+ * whole wave; exact(k, pix, r, sp, e, v0, v1) receives those inside pair k's
+ * disc and beam. */
+template <int G, bool LOAD_SKY, typename Consume, typename Exact>
+__device__ inline void batch_walk(const BatchParams &p, BatchTable<G> &bt, const Disc &d, int lane,
+                                  const Centre &c, const SampleF &sf, const BatchConsts<G> &K,
+                                  int d0, Consume &consume, Exact &exact)
+{
+    RunTable &t = bt.rt;
+    int qn = 0;
+    auto drain = [&](int base, int cnt) {
+        wave_sync();
+        if (lane < cnt) {
+            const int pix = bt.qpix[base + lane], r = bt.qring[base + lane];
+            const int k = bt.qpair[base + lane];
+            ExactVisit e;
+            float4 v0;
+            float2 v1;
+            int sp;
+            if (batch_exact_visit(p, p.pairs[d0 + k], c, bt.psi[k], pix, r, e, v0, v1, sp))
+                exact(k, pix, r, sp, e, v0, v1);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        wave_sync();
+    };
+    for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+        const int total = build_runs(p.hs, d, r0, lane, t);
+        if (total <= 0)
+            continue;
+        int s = 0;
+        int v = lane;
+        if (v < total)
+            while (v >= t.pref[s + 1])
+                s++;
+        RingRec rr = p.rings[t.ring[s]];
+        __builtin_amdgcn_s_waitcnt(0);
+        BatchPend<G> q;
+        q.live = 0;
+        q.sI = q.sQ = q.sU = 0.f;
+        q.pix = q.r = q.sp = 0;
+        for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
+            const bool act = v < total;
+            const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
+            const int r = t.ring[s];
+            const VisitRing g = visit_ring_f32(c, ring_phi(p.hs, rr, r, pix), rr.cdp, rr.sdp);
+            const int startpix = rr.startpix;
+            const int vn = v + kWaveSize;
+            if (vn < total)
+                while (vn >= t.pref[s + 1])
+                    s++;
+            rr = p.rings[t.ring[s]];
+            const VisitGeo vg = visit_geo_f32(p.fc, sf, g);
+
+            if (q.live)
+                consume(q);
+            /* the only drain site of the loop: the previous visit is consumed
+             * and the next loads are not issued yet */
+            while (qn >= kWaveSize) {
+                drain(qn - kWaveSize, kWaveSize);
+                qn -= kWaveSize;
+            }
+            /* this visit, per pair (the queue holds < 64 entries here and
+             * grows by at most G x 64 below) */
+            q.live = 0;
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+                const bool in = act && vg.omc < K.CUT_IN[k];
+                const bool edge = act && !in && !(vg.omc > K.CUT_OUT[k]);
+                float c2 = 0.f, s2 = 0.f;
+                int bp = -1;
+                if (in)
+                    bp = beam_pixel_f32_det(vg, K.T0[k], K.C2P[k], K.S2P[k], K.NSF[k], K.MK[k],
+                                            c2, s2);
+                const bool defer = edge || bp == kNeedExact;
+                const unsigned long long m = __ballot(defer);
+                if (m) {
+                    if (defer) {
+                        const int pos = qn + lanes_below(m);
+                        bt.qpix[pos] = pix;
+                        bt.qring[pos] = r;
+                        bt.qpair[pos] = k;
+                    }
+                    qn += __popcll(m);
+                }
+                const bool live = (unsigned)bp < (unsigned)K.NPIX[k];
+                q.live |= live ? (1u << k) : 0u;
+                q.qc[k] = c2;
+                q.qs[k] = s2;
+                const float *row = K.BEAM[k] + 8 * (long)(live ? bp : 0);
+                q.q0[k] = *(const float4 *)row;
+                q.q1[k] = *(const float2 *)(row + 4);
+            }
+            q.pix = pix;
+            q.r = r;
+            q.sp = startpix;
+            if (LOAD_SKY) {
+                const int sp = act ? pix : 0;
+                q.sI = p.I[sp];
+                q.sQ = p.Q[sp];
+                q.sU = p.U[sp];
+            }
+        }
+        if (q.live)
+            consume(q);
+        wave_sync();
+    }
+    while (qn > 0) {
+        const int cnt = qn < kWaveSize ? qn : kWaveSize;
+        drain(qn - cnt, cnt);
+        qn -= cnt;
+    }
 }
 
 template <int G>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_batch_fwd(BatchParams p)
 {
-    __shared__ BatchTable tabs[kWavesPerBlock];
+    __shared__ BatchTable<G> tabs[kWavesPerBlock];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
     const int lane = threadIdx.x % kWaveSize;
-    BatchTable &bt = tabs[wave];
-    RunTable &t = bt.rt;
+    BatchTable<G> &bt = tabs[wave];
 
     const long lb = xcd_remap(blockIdx.x, gridDim.x);
     const long w = lb * kWavesPerBlock + wave;
@@ -613,145 +775,40 @@ void k_batch_fwd(BatchParams p)
     SampleF sf;
     sf.cdc = unif((float)c.cdc);
     sf.sdc = unif((float)c.sdc);
-
-    /* per-pair constants of this sample: lane k computes pair d0 + k */
-    float t0l = 0.f, c2l = 1.f, s2l = 0.f;
-    if (lane < G) {
-        const float pa_k = p.pa[i] + p.pairs[d0 + lane].pa_off;   /* the pair's own pa */
-        const double psi = -(double)pa_k;
-        double s2, c2;
-        pl_sincos(2.0 * psi, &s2, &c2);
-        t0l = (float)fmodulo((kPi - psi) * kInvHalfPi, 4.0);
-        c2l = (float)c2;
-        s2l = (float)s2;
-        bt.psi[lane] = psi;
-        bt.acc[lane][0] = 0.0;
-        bt.acc[lane][1] = 0.0;
-    }
-    float T0[G], C2P[G], S2P[G], NSF[G], MK[G], CUT_IN[G], CUT_OUT[G];
-    int NPIX[G];
-    const float *BEAM[G];
-#pragma unroll
-    for (int k = 0; k < G; k++) {
-        T0[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t0l), k));
-        C2P[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c2l), k));
-        S2P[k] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, s2l), k));
-        const BatchPair &bp_ = p.pairs[d0 + k];
-        NSF[k] = bp_.nside_f;
-        MK[k] = bp_.marg_k;
-        CUT_IN[k] = bp_.omc_cut * (1.0f - 4e-6f);
-        CUT_OUT[k] = bp_.omc_cut * (1.0f + 4e-6f);
-        NPIX[k] = bp_.npix;
-        BEAM[k] = bp_.beam;
-    }
+    BatchConsts<G> K;
+    batch_consts<G, false>(p, bt, i, d0, lane, K);
     Disc d;
     disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.pairs[d0].rho_max, p.pairs[d0].cos_rmax, d);
-    wave_sync();
 
     double acc_a[G], acc_b[G];
 #pragma unroll
     for (int k = 0; k < G; k++)
         acc_a[k] = acc_b[k] = 0.0;
-    int qn = 0;
-
-    for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
-        const int total = build_runs(p.hs, d, r0, lane, t);
-        if (total <= 0)
-            continue;
-        int s = 0;
-        int v = lane;
-        if (v < total)
-            while (v >= t.pref[s + 1])
-                s++;
-        RingRec rr = p.rings[t.ring[s]];
-        __builtin_amdgcn_s_waitcnt(0);
-        /* pending visit: sky and, per pair, beam row and chi */
-        float qI = 0.f, qQ = 0.f, qU = 0.f;
-        float4 q0[G];
-        float2 q1[G];
-        float qc[G], qs[G];
-        unsigned qlive = 0;
-        for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
-            const bool act = v < total;
-            const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
-            const int r = t.ring[s];
-            const VisitRing g = visit_ring_f32(c, ring_phi(p.hs, rr, r, pix), rr.cdp, rr.sdp);
-            const int vn = v + kWaveSize;
-            if (vn < total)
-                while (vn >= t.pref[s + 1])
-                    s++;
-            rr = p.rings[t.ring[s]];
-            const VisitGeo vg = visit_geo_f32(p.fc, sf, g);
-
-            /* sums of the previous visit */
-#pragma unroll
-            for (int k = 0; k < G; k++) {
-                if (qlive & (1u << k)) {
-                    const float A0 = q0[k].x, A1 = q0[k].y, A2 = q0[k].z, B0 = q0[k].w;
-                    const float B1 = q1[k].x, B2 = q1[k].y, cc = qc[k], ss = qs[k];
-                    acc_a[k] += qI * A0 + qQ * (A1 * cc - A2 * ss) + qU * (A2 * cc + A1 * ss);
-                    acc_b[k] += qI * B0 + qQ * (-B1 * cc + B2 * ss) + qU * (-B2 * cc - B1 * ss);
-                }
-            }
-            /* the only drain site of the loop: the previous visit is consumed
-             * and the next loads are not issued yet (little is live across the
-             * exact path's out-of-line call) */
-            while (qn >= kWaveSize) {
-                batch_drain<G>(p, bt, d0, qn - kWaveSize, kWaveSize, lane, c);
-                qn -= kWaveSize;
-            }
-
-            /* this visit, per pair (the queue holds < 64 entries here and
-             * grows by at most G x 64 below) */
-            qlive = 0;
-#pragma unroll
-            for (int k = 0; k < G; k++) {
-                const bool in = act && vg.omc < CUT_IN[k];
-                const bool edge = act && !in && !(vg.omc > CUT_OUT[k]);
-                float c2 = 0.f, s2 = 0.f;
-                int bp = -1;
-                if (in)
-                    bp = beam_pixel_f32_det(vg, T0[k], C2P[k], S2P[k], NSF[k], MK[k], c2, s2);
-                const bool defer = edge || bp == kNeedExact;
-                const unsigned long long m = __ballot(defer);
-                if (m) {
-                    if (defer) {
-                        const int pos = qn + lanes_below(m);
-                        bt.qpix[pos] = pix;
-                        bt.qring[pos] = r;
-                        bt.qpair[pos] = k;
-                    }
-                    qn += __popcll(m);
-                }
-                const bool live = (unsigned)bp < (unsigned)NPIX[k];
-                qlive |= live ? (1u << k) : 0u;
-                qc[k] = c2;
-                qs[k] = s2;
-                const float *row = BEAM[k] + 8 * (long)(live ? bp : 0);
-                q0[k] = *(const float4 *)row;
-                q1[k] = *(const float2 *)(row + 4);
-            }
-            const int sp = act ? pix : 0;
-            qI = p.I[sp];
-            qQ = p.Q[sp];
-            qU = p.U[sp];
-        }
+    auto consume = [&](const BatchPend<G> &q) {
 #pragma unroll
         for (int k = 0; k < G; k++) {
-            if (qlive & (1u << k)) {
-                const float A0 = q0[k].x, A1 = q0[k].y, A2 = q0[k].z, B0 = q0[k].w;
-                const float B1 = q1[k].x, B2 = q1[k].y, cc = qc[k], ss = qs[k];
-                acc_a[k] += qI * A0 + qQ * (A1 * cc - A2 * ss) + qU * (A2 * cc + A1 * ss);
-                acc_b[k] += qI * B0 + qQ * (-B1 * cc + B2 * ss) + qU * (-B2 * cc - B1 * ss);
+            if (q.live & (1u << k)) {
+                const float A0 = q.q0[k].x, A1 = q.q0[k].y, A2 = q.q0[k].z, B0 = q.q0[k].w;
+                const float B1 = q.q1[k].x, B2 = q.q1[k].y, cc = q.qc[k], ss = q.qs[k];
+                acc_a[k] += q.sI * A0 + q.sQ * (A1 * cc - A2 * ss) + q.sU * (A2 * cc + A1 * ss);
+                acc_b[k] += q.sI * B0 + q.sQ * (-B1 * cc + B2 * ss) + q.sU * (-B2 * cc - B1 * ss);
             }
         }
-        wave_sync();
-    }
-    while (qn > 0) {
-        const int cnt = qn < kWaveSize ? qn : kWaveSize;
-        batch_drain<G>(p, bt, d0, qn - cnt, cnt, lane, c);
-        qn -= cnt;
-    }
+    };
+    auto exact = [&](int k, int pix, int r, int sp, const ExactVisit &e, const float4 &v0,
+                     const float2 &v1) {
+        (void)r;
+        (void)sp;
+        const float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
+        const float ta =
+            sI * v0.x + sQ * (v0.y * e.c2 - v0.z * e.s2) + sU * (v0.z * e.c2 + v0.y * e.s2);
+        const float tb =
+            sI * v0.w + sQ * (-v1.x * e.c2 + v1.y * e.s2) + sU * (-v1.y * e.c2 - v1.x * e.s2);
+        atomicAdd(&bt.acc[k][0], (double)ta);
+        atomicAdd(&bt.acc[k][1], (double)tb);
+    };
+    batch_walk<G, true>(p, bt, d, lane, c, sf, K, d0, consume, exact);
+    wave_sync();
 #pragma unroll
     for (int k = 0; k < G; k++) {
         const double ta = wave_sum(acc_a[k]) + bt.acc[k][0];
@@ -807,12 +864,31 @@ __device__ inline void adj_visit_values(float ga, float gb, float4 v0, float2 v1
     vU = ga * (A2 * c2 + A1 * s2) + gb * (-B2 * c2 - B1 * s2);
 }
 
-template <bool BATCH>
+/* Per-wave LDS table of the sky adjoint: a run table, or the batch table of
+ * a group of BG >= 2 pairs. */
+template <int BG>
+struct AdjWaveTable {
+    using type = BatchTable<BG>;
+};
+template <>
+struct AdjWaveTable<0> {
+    using type = RunTable;
+};
+template <>
+struct AdjWaveTable<1> {
+    using type = RunTable;
+};
+
+/* BG: 0 one pair; 1 a batch, one wave per (pair, sample); 2, 4 a batch, one
+ * wave per (group of BG pairs, sample) sharing each visit's geometry. */
+template <int BG>
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
 {
+    constexpr bool BATCH = BG > 0;
+    using WT = typename AdjWaveTable<BG>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    RunTable *runs = (RunTable *)smem;
-    int *win_lo = (int *)(runs + kAdjWaves);
+    WT *wtab = (WT *)smem;
+    int *win_lo = (int *)(wtab + kAdjWaves);
     int *win_hi = win_lo + L.rw;
     int *win_base = win_hi + L.rw;          /* rw + 1 */
     int *ctl = win_base + L.rw + 1;         /* 0 R0, 1 R1, 2 W, 3 fallback */
@@ -821,7 +897,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWaveSize);
     const int lane = tid % kWaveSize;
-    RunTable &t = runs[wave];
+    RunTable &t = *(RunTable *)&wtab[wave];   /* the first member of either table */
 
     /* Window radius.  A visit adds a non-zero value only if its beam pixel
      * is at or before row `last`, i.e. lies in a beam ring <= R of that row.
@@ -958,8 +1034,79 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             __syncthreads();
         }
 
-        /* pass 2: visits, one wave per (pair, sample), pairs outermost */
+        /* pass 2: visits; adds I,Q,U of one visit to the LDS window (or, when no
+         * window holds the pixel, to the global FP64 gradient) */
         const int W = fallback ? 0 : ctl[2];
+        auto scatter = [&](int pix, int r, int sp, float vI, float vQ, float vU) {
+            if (vI == 0.f && vQ == 0.f && vU == 0.f)
+                return;   /* adds nothing */
+            const int k = r - R0;
+            int slot = -1;
+            if (!fallback && (unsigned)k < (unsigned)nring) {
+                int nr = 4 * (r < p.hs.nside ? r
+                              : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r : p.hs.nside));
+                int off = pix - sp - win_lo[k];
+                if (off < 0)
+                    off += nr;
+                if (off < win_hi[k])   /* inside the ring's window */
+                    slot = win_base[k] + off;
+            }
+            if (slot >= 0) {
+                atomicAdd(&acc[slot], vI);
+                atomicAdd(&acc[W + slot], vQ);
+                atomicAdd(&acc[2 * W + slot], vU);
+            } else {   /* no window holds the pixel: global FP64 atomics */
+                unsafeAtomicAdd(&p.gI[pix], (double)vI);
+                unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
+                unsafeAtomicAdd(&p.gU[pix], (double)vU);
+            }
+        };
+        if constexpr (BG >= 2) {
+            /* one wave per (group, sample), groups outermost */
+            BatchTable<BG> &bt = wtab[wave];
+            const int ngr = L.npairs / BG;
+            for (int j = wave; j < ns * ngr; j += kAdjWaves) {
+                const int jg = j / ns;
+                const long i = s0 + (j - jg * ns);
+                const int d0 = jg * BG;
+                Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+                c.ra = uni(c.ra);
+                c.dec = uni(c.dec);
+                c.cdc = uni(c.cdc);
+                c.sdc = uni(c.sdc);
+                SampleF sf;
+                sf.cdc = unif((float)c.cdc);
+                sf.sdc = unif((float)c.sdc);
+                BatchConsts<BG> K;
+                batch_consts<BG, true>(L.b, bt, i, d0, lane, K);
+                Disc d;
+                disc_setup(p.hs, kHalfPi - c.dec, c.ra, L.pairs[d0].rho_max,
+                           L.pairs[d0].cos_rmax, d);
+                auto consume = [&](const BatchPend<BG> &q) {
+                    float vI = 0.f, vQ = 0.f, vU = 0.f;
+#pragma unroll
+                    for (int k = 0; k < BG; k++) {
+                        if (q.live & (1u << k)) {
+                            float dI, dQ, dU;
+                            adj_visit_values(K.GA[k], K.GB[k], q.q0[k], q.q1[k], q.qc[k],
+                                             q.qs[k], dI, dQ, dU);
+                            vI += dI;
+                            vQ += dQ;
+                            vU += dU;
+                        }
+                    }
+                    scatter(q.pix, q.r, q.sp, vI, vQ, vU);
+                };
+                auto exact = [&](int k, int pix, int r, int sp, const ExactVisit &e,
+                                 const float4 &v0, const float2 &v1) {
+                    float vI, vQ, vU;
+                    adj_visit_values(bt.ga[k], bt.gb[k], v0, v1, e.c2, e.s2, vI, vQ, vU);
+                    scatter(pix, r, sp, vI, vQ, vU);
+                };
+                batch_walk<BG, false>(L.b, bt, d, lane, c, sf, K, d0, consume, exact);
+            }
+        } else {
+        /* one wave per (pair, sample), pairs outermost */
         const int np = BATCH ? L.npairs : 1;
         for (int j = wave; j < ns * np; j += kAdjWaves) {
             const int jp = j / ns;
@@ -984,31 +1131,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             const float ga = unif(pp.g_a ? pp.g_a[i] : 0.f);
             const float gb = unif(pp.g_b ? pp.g_b[i] : 0.f);
             auto use = [&](const Pending &q) {
-                const int pix = q.pix, r = q.r;
                 float vI, vQ, vU;
                 adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
-                if (vI == 0.f && vQ == 0.f && vU == 0.f)
-                    return;   /* adds nothing */
-                const int k = r - R0;
-                int slot = -1;
-                if (!fallback && (unsigned)k < (unsigned)nring) {
-                    int nr = 4 * (r < pp.hs.nside ? r
-                                  : (r > 3 * pp.hs.nside ? 4 * pp.hs.nside - r : pp.hs.nside));
-                    int off = pix - q.sp - win_lo[k];
-                    if (off < 0)
-                        off += nr;
-                    if (off < win_hi[k])   /* inside the ring's window */
-                        slot = win_base[k] + off;
-                }
-                if (slot >= 0) {
-                    atomicAdd(&acc[slot], vI);
-                    atomicAdd(&acc[W + slot], vQ);
-                    atomicAdd(&acc[2 * W + slot], vU);
-                } else {   /* no window holds the pixel: global FP64 atomics */
-                    unsafeAtomicAdd(&pp.gI[pix], (double)vI);
-                    unsafeAtomicAdd(&pp.gQ[pix], (double)vQ);
-                    unsafeAtomicAdd(&pp.gU[pix], (double)vU);
-                }
+                scatter(q.pix, q.r, q.sp, vI, vQ, vU);
             };
             int qn = 0;
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
@@ -1018,6 +1143,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             }
             if (qn > 0)
                 drain_exact<false>(pp, t, 0, qn, lane, c, use);
+        }
         }
         __syncthreads();
 
@@ -1290,14 +1416,33 @@ hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
-size_t adjmap_lds_bytes(int cap, int rw)
+size_t adjmap_lds_bytes(int cap, int rw, int group)
 {
-    return sizeof(RunTable) * kAdjWaves + sizeof(int) * (3 * (size_t)rw + 1 + 4) +
+    const size_t wt = group == 2 ? sizeof(BatchTable<2>)
+                      : group == 4 ? sizeof(BatchTable<4>)
+                                   : sizeof(RunTable);
+    return wt * kAdjWaves + sizeof(int) * (3 * (size_t)rw + 1 + 4) +
            sizeof(float) * 3 * (size_t)cap;
 }
 
+template <int BG>
+static hipError_t launch_adjmap_bg(const PairParams &p, const AdjLayout &L, size_t bytes,
+                                   long g, hipStream_t st)
+{
+    static size_t configured = 0;
+    if (bytes > configured) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess)
+            return e;
+        configured = bytes;
+    }
+    hipLaunchKernelGGL(k_adjmap_lds<BG>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
-                             hipStream_t st, const BatchParams *batch, double rho_w)
+                             hipStream_t st, const BatchParams *batch, double rho_w, int group)
 {
     long n = p.s_end - p.s_begin;
     if (n <= 0)
@@ -1315,35 +1460,29 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
         L.cos_w = pl_cos(rho_w);
         L.b = *batch;
     }
-    size_t bytes = adjmap_lds_bytes(cap, rw);
-    static size_t configured[2] = {0, 0};
-    const void *fn = batch ? (const void *)k_adjmap_lds<true> : (const void *)k_adjmap_lds<false>;
-    if (bytes > configured[batch ? 1 : 0]) {
-        hipError_t e =
-            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess)
-            return e;
-        configured[batch ? 1 : 0] = bytes;
+    const int bg = batch ? group : 0;
+    const size_t bytes = adjmap_lds_bytes(cap, rw, bg);
+    const long g = std::min<long>(L.nchunks, nblocks);
+    switch (bg) {
+    case 0: return launch_adjmap_bg<0>(p, L, bytes, g, st);
+    case 1: return launch_adjmap_bg<1>(p, L, bytes, g, st);
+    case 2: return launch_adjmap_bg<2>(p, L, bytes, g, st);
+    case 4: return launch_adjmap_bg<4>(p, L, bytes, g, st);
+    default: return hipErrorInvalidValue;
     }
-    long g = std::min<long>(L.nchunks, nblocks);
-    if (batch)
-        hipLaunchKernelGGL(k_adjmap_lds<true>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
-    else
-        hipLaunchKernelGGL(k_adjmap_lds<false>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
-    return hipGetLastError();
 }
 
 int adjmap_describe(char *buf, size_t n, int cap, int rw)
 {
     hipFuncAttributes a;
-    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds<false>);
+    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds<0>);
     if (e != hipSuccess)
         return snprintf(buf, n, "hipFuncGetAttributes: %s", hipGetErrorString(e));
     return snprintf(buf, n,
                     "block %d threads, kernel max %d, regs %d, static LDS %zu, dynamic LDS "
                     "%zu (max %d), private %zu",
                     kAdjBlock, a.maxThreadsPerBlock, a.numRegs, a.sharedSizeBytes,
-                    adjmap_lds_bytes(cap, rw), a.maxDynamicSharedSizeBytes, a.localSizeBytes);
+                    adjmap_lds_bytes(cap, rw, 0), a.maxDynamicSharedSizeBytes, a.localSizeBytes);
 }
 
 hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st)

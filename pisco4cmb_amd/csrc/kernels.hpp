@@ -82,8 +82,9 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st, const BatchParams *batch = nullptr,
-                             double rho_w = 0.0);
-size_t adjmap_lds_bytes(int cap, int rw);
+                             double rho_w = 0.0, int group = 1);
+/* group: 0 one pair, 1 per-pair batch items, 2 or 4 pairs per wave */
+size_t adjmap_lds_bytes(int cap, int rw, int group = 0);
 /* Launch resources of the sky-adjoint kernel, for error messages. */
 int adjmap_describe(char *buf, size_t n, int cap, int rw);
 /* *last = index of the last row of beam8 [npix][8] with a non-zero value in

@@ -67,7 +67,8 @@ def test_batched_forward_matches_single_pair_and_oracle(case, oracle, group):
     batch.close()
 
 
-def test_batched_adjoint_sum_of_pairs_and_dot_product(case):
+@pytest.mark.parametrize("adj_group", [1, 2, 4])
+def test_batched_adjoint_sum_of_pairs_and_dot_product(case, adj_group):
     """Batched sky adjoint = sum over pairs of the single-pair adjoints
     (relative RMS 1e-6); <tod, g> = <sky, A^T g> to 1e-6 through the
     autograd binding."""
@@ -77,6 +78,8 @@ def test_batched_adjoint_sum_of_pairs_and_dot_product(case):
     rat, dect, pat = t(ra), t(dec), t(pa)
     specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
     batch = DetectorBatch(specs)
+    ctx = torch_ops._ctx(torch.device("cuda:0"))
+    check(lib.pisco_set_tuning(ctx, b"batch_adj_group", adj_group))
     rng = np.random.default_rng(8)
     g = t(rng.normal(size=(len(pairs), 2, ra.size)).astype(np.float32))
     tod = batched_convolve(I, Q, U, batch, rat, dect, pat, NS)
@@ -97,4 +100,5 @@ def test_batched_adjoint_sum_of_pairs_and_dot_product(case):
     lhs = float(L.item())
     rhs = sum(float(np.dot(got[c].astype(np.float64), sky[c])) for c in range(3))
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+    check(lib.pisco_set_tuning(ctx, b"batch_adj_group", 4))
     batch.close()
