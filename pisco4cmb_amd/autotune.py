@@ -1,0 +1,69 @@
+"""Launch-shape autotuner: times the candidate tile shapes of a workload on
+the device and keeps the fastest (pisco_set_tuning).
+
+    best = autotune_batch(batch, I, Q, U, ra, dec, pa, nside_sky)
+
+Candidates: pairs per wave of the batched forward (batch_group 1, 2, 4) and
+of the batched sky adjoint (batch_adj_group 1, 2, 4), and samples per LDS
+window of the sky adjoint (adjmap_chunk 8, 16, 32).  The timing runs on the
+first `sample` samples of the caller's own pointing (HIP events on torch's
+current stream); results do not depend on the choice, only the time does.
+"""
+import ctypes
+
+import torch
+
+from ._lib import check, lib
+from .torch_ops import _ctx
+
+
+def _time(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps / 1e3
+
+
+def _set(ctx, key, value):
+    check(lib.pisco_set_tuning(ctx, key.encode(), int(value)))
+
+
+def _get(ctx, key):
+    v = ctypes.c_long()
+    check(lib.pisco_get_tuning(ctx, key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def autotune_batch(batch, I, Q, U, ra, dec, pa, nside_sky, sample=65536, reps=2):
+    """Picks batch_group, batch_adj_group and adjmap_chunk for a
+    DetectorBatch on this pointing; returns {key: (best, {value: seconds})}."""
+    ctx = _ctx(batch.device)
+    n = min(sample, ra.numel())
+    r, d, p = ra[:n], dec[:n], pa[:n]
+    g = torch.randn((batch.npairs, 2, n), device=batch.device)
+    result = {}
+    for key, values, fn in (
+            ("batch_group", (1, 2, 4), lambda: batch.forward(I, Q, U, r, d, p, nside_sky)),
+            ("batch_adj_group", (1, 2, 4), lambda: batch.adjoint_map(g, r, d, p, nside_sky)),
+            ("adjmap_chunk", (8, 16, 32), lambda: batch.adjoint_map(g, r, d, p, nside_sky))):
+        times = {}
+        for v in values:
+            _set(ctx, key, v)
+            times[v] = _time(fn, reps)
+        best = min(times, key=times.get)
+        _set(ctx, key, best)
+        result[key] = (best, times)
+    return result
+
+
+def tuning(device=None):
+    """The current launch shapes of the device's context."""
+    ctx = _ctx(device or torch.device("cuda", torch.cuda.current_device()))
+    return {k: _get(ctx, k) for k in ("batch_group", "batch_adj_group", "adjmap_chunk",
+                                      "adjmap_cap", "adjmap_rw", "adjmap_support")}

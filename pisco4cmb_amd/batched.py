@@ -106,3 +106,25 @@ class BatchedConvolution(torch.autograd.Function):
 def batched_convolve(I, Q, U, batch, ra, dec, pa, nside_sky):
     """Differentiable TOD [npairs, 2, nsamples] of a DetectorBatch."""
     return BatchedConvolution.apply(I, Q, U, batch, ra, dec, pa, nside_sky)
+
+
+class BatchedStepGraph:
+    """One batched forward + sky-adjoint step (every scan chunk) captured in a
+    hipGraph (torch.cuda.CUDAGraph) and replayed: tod = forward(sky) and
+    (gI, gQ, gU) = adjoint(g) on fixed input tensors, whose contents may
+    change between replays.  A warm-up call runs first, so no allocation or
+    first-use setup happens during capture."""
+
+    def __init__(self, batch, I, Q, U, ra, dec, pa, g, nside_sky):
+        self.inputs = (I, Q, U, ra, dec, pa, g)
+        batch.forward(I, Q, U, ra, dec, pa, nside_sky)
+        batch.adjoint_map(g, ra, dec, pa, nside_sky)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.tod = batch.forward(I, Q, U, ra, dec, pa, nside_sky)
+            self.grad = batch.adjoint_map(g, ra, dec, pa, nside_sky)
+
+    def replay(self):
+        self.graph.replay()
+        return self.tod, self.grad

@@ -102,3 +102,48 @@ def test_batched_adjoint_sum_of_pairs_and_dot_product(case, adj_group):
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
     check(lib.pisco_set_tuning(ctx, b"batch_adj_group", 4))
     batch.close()
+
+
+def test_graph_replays_match_eager(case):
+    """A captured batched forward + adjoint step replays to the eager results
+    (three replays with eager work and synchronisations in between, the sky
+    changed before the last one)."""
+    from pisco4cmb_amd.batched import BatchedStepGraph
+    sky, pairs, ra, dec, pa = case
+    I, Q, U = (t(x) for x in sky[:3])
+    rat, dect, pat = t(ra), t(dec), t(pa)
+    specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
+    batch = DetectorBatch(specs)
+    g = torch.randn((len(pairs), 2, ra.size), device="cuda")
+    step = BatchedStepGraph(batch, I, Q, U, rat, dect, pat, g, NS)
+    for rep in range(3):
+        if rep == 2:
+            I.mul_(0.5)
+        tod, grad = step.replay()
+        torch.cuda.synchronize()
+        got_tod = tod.cpu().numpy()
+        got_grad = [x.cpu().numpy() for x in grad]
+        ref_tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
+        ref_grad = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
+        torch.cuda.synchronize()
+        assert rel_rms(got_tod, ref_tod) < 1e-6
+        for a, b in zip(got_grad, ref_grad):
+            assert rel_rms(a, b) < 1e-6
+    batch.close()
+
+
+def test_autotuner_picks_a_valid_shape(case):
+    from pisco4cmb_amd.autotune import autotune_batch, tuning
+    sky, pairs, ra, dec, pa = case
+    I, Q, U = (t(x) for x in sky[:3])
+    specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
+    batch = DetectorBatch(specs)
+    res = autotune_batch(batch, I, Q, U, t(ra), t(dec), t(pa), NS, sample=1024, reps=1)
+    cur = tuning()
+    for key, (best, times) in res.items():
+        assert best in times and cur[key] == best
+    from pisco4cmb_amd._lib import check as _check
+    ctx = torch_ops._ctx(torch.device("cuda:0"))
+    for k, v in (("batch_group", 4), ("batch_adj_group", 4), ("adjmap_chunk", 16)):
+        _check(lib.pisco_set_tuning(ctx, k.encode(), v))
+    batch.close()
