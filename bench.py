@@ -21,11 +21,12 @@ with |D_i| the exact disc sizes (device query_disc).  achieved = bytes of
 the dominant kernel / its mean duration (HIP events on the launch stream).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
-                  [--no-cpu] [--no-forms] [--calib-copy-mib M]
+                  [--no-cpu] [--no-forms] [--no-batched] [--calib-copy-mib M]
 
 Also reported: the plain drop-in forward of the NSIDE 256 / 1.5 deg workload
-(device-resident and through the host API including PCIe copies), and the
-CPU oracle on a prefix of the headline workload.
+(device-resident and through the host API including PCIe copies), the
+batched 64-detector forward + adjoint against the plain per-pair launches,
+and the CPU oracle on a prefix of the headline workload.
 """
 import argparse
 import ctypes
@@ -68,6 +69,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-forms", action="store_true",
                    help="skip the plain drop-in forward of the NSIDE 256 workload")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the batched 64-detector workload")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="run one device-to-device copy of this size first (a kernel of "
                         "known traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -203,6 +206,8 @@ def main():
 
     if not args.no_forms:
         out["plain_forward_nside256"] = plain_forward(ctx, stream, dev)
+    if not args.no_batched:
+        out["batched_64det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra,
                                            dec, pa, g)
@@ -275,6 +280,99 @@ def plain_forward(ctx, stream, dev, n=1_000_000, reps=5):
             "device_resident": {"tod_samples_per_s": 2 * n / t_dev, "s": t_dev,
                                 "hbm_frac": b / t_dev / HBM_PEAK},
             "host_api_with_pcie": {"tod_samples_per_s": 2 * n / t_host, "s": t_host}}
+
+
+def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=32, reps=3):
+    """64 detectors (32 pairs), each pair with its own elliptical Gaussian beam
+    (FWHM U[0.2, 1.0] deg, ellipticity U[0, 0.1], cap 3 x FWHM, beam NSIDE
+    2048) and position-angle offset, on the headline sky and a 2.5e5-sample
+    CES scan: the batched forward + sky adjoint (one launch each, the step
+    replayed from a hipGraph) against the plain form (one single-pair launch
+    per pair and operation)."""
+    import numpy as np
+    import torch
+
+    from pisco4cmb_amd import inputs, torch_ops
+    from pisco4cmb_amd._lib import check, lib
+    from pisco4cmb_amd.batched import BatchedStepGraph, DetectorBatch
+    rng = np.random.default_rng(64)
+    ra, dec, pa = inputs.ces_scan(n, t0=3600.0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rat, dect, pat = t(ra), t(dec), t(pa)
+    specs, radii = [], []
+    for _ in range(npairs):
+        fwhm = float(rng.uniform(0.2, 1.0))
+        b = inputs.gaussian_polbeam(2048, 3 * fwhm, fwhm, float(rng.uniform(0, 0.1)), 0.0,
+                                    nside_sky=nside_sky)
+        specs.append((t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), 2048,
+                      float(rng.uniform(-np.pi, np.pi))))
+        radii.append(b.get_rho_max())
+    batch = DetectorBatch(specs)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    visits = 0
+    for r in radii:
+        check(lib.pisco_dev_query_disc(ctx, stream, nside_sky, n, rat.data_ptr(),
+                                       dect.data_ptr(), r, None, counts.data_ptr(), None))
+        visits += int(counts.to(torch.int64).sum().item())
+    g = torch.randn((npairs, 2, n), device=dev)
+
+    def tm(f):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_bf = tm(lambda: batch.forward(It, Qt, Ut, rat, dect, pat, nside_sky))
+    t_ba = tm(lambda: batch.adjoint_map(g, rat, dect, pat, nside_sky))
+    step = BatchedStepGraph(batch, It, Qt, Ut, rat, dect, pat, g, nside_sky)
+    t_step = tm(step.replay)
+    pas = [t((pa + np.float32(sp[3])).astype(np.float32)) for sp in specs]
+    b8 = [torch_ops.pack_beam(sp[0], sp[1]) for sp in specs]
+    ta = torch.empty(n, device=dev)
+    tb = torch.empty(n, device=dev)
+    gI = torch.empty(12 * nside_sky ** 2, device=dev)
+    gQ = torch.empty_like(gI)
+    gU = torch.empty_like(gI)
+
+    def plain_fwd():
+        for k in range(npairs):
+            check(lib.pisco_dev_forward(ctx, stream, nside_sky, It.data_ptr(), Qt.data_ptr(),
+                                        Ut.data_ptr(), 2048, b8[k].shape[0], b8[k].data_ptr(),
+                                        n, rat.data_ptr(), dect.data_ptr(), pas[k].data_ptr(),
+                                        ta.data_ptr(), tb.data_ptr()))
+
+    def plain_adj():
+        for k in range(npairs):
+            check(lib.pisco_dev_adjoint_map(ctx, stream, nside_sky, 2048, b8[k].shape[0],
+                                            b8[k].data_ptr(), n, rat.data_ptr(),
+                                            dect.data_ptr(), pas[k].data_ptr(),
+                                            g[k, 0].data_ptr(), g[k, 1].data_ptr(),
+                                            gI.data_ptr(), gQ.data_ptr(), gU.data_ptr()))
+
+    t_pf = tm(plain_fwd)
+    t_pa = tm(plain_adj)
+    batch.close()
+    det = 2 * npairs * n
+    b_fa = visits * (36 + 48) + 2 * npairs * n * 20
+    return {"workload": "%d detectors (%d pairs), FWHM U[0.2,1.0] deg, cap 3xFWHM, beam NSIDE "
+                        "2048, sky NSIDE %d, %d samples CES, per-pair pa offsets"
+                        % (2 * npairs, npairs, nside_sky, n),
+            "pair_visits": visits,
+            "batched_fwd_adj_graph": {"det_samples_per_s": det / t_step, "s": t_step,
+                                      "hbm_frac": b_fa / t_step / HBM_PEAK},
+            "batched_forward": {"det_samples_per_s": det / t_bf, "s": t_bf},
+            "batched_adjoint": {"det_samples_per_s": det / t_ba, "s": t_ba},
+            "plain_forward": {"det_samples_per_s": det / t_pf, "s": t_pf},
+            "plain_fwd_adj": {"det_samples_per_s": det / (t_pf + t_pa), "s": t_pf + t_pa,
+                              "hbm_frac": b_fa / (t_pf + t_pa) / HBM_PEAK},
+            "batched_fwd_adj_vs_plain_forward": t_pf / t_step,
+            "batched_fwd_adj_vs_plain_fwd_adj": (t_pf + t_pa) / t_step}
 
 
 def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g):
