@@ -197,6 +197,63 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *ctx, void *stream, const pisco_batc
                                   const float *dec, const float *pa, const float *g,
                                   float *gI, float *gQ, float *gU);
 
+/* ---- binned map-maker (SkyMap) ------------------------------------------ */
+
+/* The normal equations AtA (3x3) and AtD (3) of every pixel of a RING map,
+ * held on one device; SkyMap (src/SkyMap/skymap.hpp:13-55). */
+typedef struct pisco_skymap pisco_skymap;
+
+/* SkyMap::SkyMap(int nside) (skymap.cpp:21-39).  The accumulators start at
+ * zero (the reference's zeroing, skymap.cpp:26-38, is commented out and its
+ * sums start from uninitialised memory). */
+int pisco_skymap_create(int device, int nside, pisco_skymap **out);
+void pisco_skymap_destroy(pisco_skymap *map);
+/* Zero the accumulators (a fresh SkyMap). */
+int pisco_skymap_reset(pisco_skymap *map);
+/* Samples per accumulation pass (default 2^24; the sums do not depend on
+ * it).  Scratch is 8 bytes per sample of a pass. */
+int pisco_skymap_set_chunk(pisco_skymap *map, long samples);
+
+/* SkyMap::accumulate_data (skymap.cpp:68-132), host buffers.  ra, dec, pa
+ * (float radians, pa counter-clockwise) and data hold ndet * nsamples floats,
+ * detector-major; det_pol_angles holds ndet.  Adds onto the current sums in
+ * the reference's order (detector, then sample), so the sums equal its
+ * sequential FP64 sums bit for bit.  A sample whose theta = pi/2 - dec falls
+ * outside [0, pi] (Healpix_Base::ang2pix's planck_assert) or whose ra is not
+ * finite is skipped; every other sample is accumulated and the call then
+ * returns PISCO_ERR_INVALID. */
+int pisco_skymap_accumulate(pisco_skymap *map, long nsamples, const float *ra,
+                            const float *dec, const float *pa, int ndet,
+                            const float *det_pol_angles, const float *data);
+/* Device-resident form on `stream` (NULL: HIP's null stream).  All pointers
+ * are device memory.  shared_pointing = 1: ra, dec and pa hold nsamples
+ * floats used by every detector (the sums of passing them replicated).
+ * Skipped samples are counted (pisco_skymap_skipped), not reported.  Calls
+ * on one map share its scratch: they must not overlap in time. */
+int pisco_dev_skymap_accumulate(pisco_skymap *map, void *stream, long nsamples,
+                                const float *ra, const float *dec, const float *pa,
+                                int shared_pointing, int ndet, const float *det_pol_angles,
+                                const float *data);
+/* Samples skipped since the map was created or reset (synchronises the
+ * device). */
+int pisco_skymap_skipped(pisco_skymap *map, long *count);
+
+/* SkyMap::solve_map (skymap.cpp:134-231) into host arrays of 12 nside^2:
+ * pixels with at least 3 hits get the solution of AtA x = AtD by LAPACK
+ * DGESV's arithmetic (Netlib DGETF2 + DGETRS); a singular system gives
+ * I, Q, U = AtD and hits 0; fewer than 3 hits give 0, 0, 0 and the hits.
+ * singular (nullable) receives the number of singular pixels, which the
+ * reference prints one by one. */
+int pisco_skymap_solve(pisco_skymap *map, float *I, float *Q, float *U, int *hits,
+                       long *singular);
+/* Device-resident form: I, Q, U, hits are device arrays; singular as above
+ * (synchronises the stream when non-NULL). */
+int pisco_dev_skymap_solve(pisco_skymap *map, void *stream, float *I, float *Q, float *U,
+                           int *hits, long *singular);
+/* The sums in the reference's storage (skymap.hpp:42-43): AtA [npix][9]
+ * row-major 3x3, AtD [npix][3], host arrays. */
+int pisco_skymap_get_normal(pisco_skymap *map, double *AtA, double *AtD);
+
 /* ---- HEALPix / matching primitives on the device (for verification) ---- */
 
 /* Healpix_Base::pix2ang for n pixels (device arrays). */

@@ -937,3 +937,172 @@ long or_check_cap_division(int nmax)
     }
     return bad;
 }
+
+/* ------------------------------------------------------------------ */
+/* Binned map-maker: SkyMap (src/SkyMap/skymap.cpp)                    */
+/* ------------------------------------------------------------------ */
+
+/* SkyMap::accumulate_data (skymap.cpp:68-116), with the reference's storage:
+ * AtA[9 * pix + k] row-major 3x3 and AtD[3 * pix + k].  ra, dec, pa and data
+ * are detector-major [ndet][nsamples]; ptg_stride is nsamples there, or 0 when
+ * every detector shares one pointing (the same sums as passing the pointing
+ * replicated per detector).  Accumulates onto what AtA and AtD hold.  Returns
+ * the number of samples skipped: a theta outside [0, pi], where
+ * Healpix_Base::ang2pix fails its planck_assert, or a non-finite ra, for which
+ * it would return an arbitrary index. */
+long or_skymap_accumulate(int nside, long nsamples, const float *ra, const float *dec,
+                          const float *pa, long ptg_stride, int ndet,
+                          const float *det_pol_angles, const float *data, double *AtA,
+                          double *AtD)
+{
+    hpx_t h = hpx_make(nside);
+    long skipped = 0;
+    for (int det = 0; det < ndet; det++) {
+        double pol_angle = det_pol_angles[det];
+        for (long s = 0; s < nsamples; s++) {
+            long ip = det * ptg_stride + s;
+            double phi = ra[ip];
+            double theta = O_HALFPI - (double)dec[ip];
+            double psi = -(pol_angle + (double)pa[ip]);
+            if (!(theta >= 0.0 && theta <= O_PI && fabs(phi) < INFINITY)) {
+                skipped++;
+                continue;
+            }
+            long p = hpx_ang2pix(&h, theta, phi);
+            double d = data[det * nsamples + s];
+            double twopsi = 2.0 * psi;
+            double c = O_COS(twopsi);
+            double sn = O_SIN(twopsi);
+            AtD[p * 3 + 0] += 1.0 * d;
+            AtD[p * 3 + 1] += c * d;
+            AtD[p * 3 + 2] += sn * d;
+            AtA[p * 9 + 0] += 1.0;
+            AtA[p * 9 + 1] += c;
+            AtA[p * 9 + 2] += sn;
+            AtA[p * 9 + 3] += c;
+            AtA[p * 9 + 4] += c * c;
+            AtA[p * 9 + 5] += c * sn;
+            AtA[p * 9 + 6] += sn;
+            AtA[p * 9 + 7] += c * sn;
+            AtA[p * 9 + 8] += sn * sn;
+        }
+    }
+    return skipped;
+}
+
+/* LAPACK DGESV for n = 3, nrhs = 1, as the Netlib reference implementation
+ * computes it: DGETF2 (right-looking LU, partial pivoting with IDAMAX's first
+ * maximum, the column below the pivot scaled by the reciprocal of the pivot
+ * when |pivot| >= DLAMCH('S'), rank-1 update by DGER, which skips a column
+ * whose multiplier row entry is zero), then DGETRS (DLASWP, then DTRSM lower
+ * unit and upper non-unit, each skipping a zero right-hand-side entry).
+ * a is column-major a[i + 3 j]; on return b holds the solution unless the
+ * result is non-zero (an exactly zero pivot: b is then left as it was, as
+ * DGESV leaves it). */
+static int o_dgesv3(double a[9], double b[3])
+{
+    const int n = 3;
+    const double sfmin = 2.2250738585072014e-308;   /* DLAMCH('S') */
+    int ipiv[3];
+    int info = 0;
+    for (int j = 0; j < n; j++) {
+        int jp = j;
+        double amax = fabs(a[j + 3 * j]);
+        for (int i = j + 1; i < n; i++)
+            if (fabs(a[i + 3 * j]) > amax) {
+                amax = fabs(a[i + 3 * j]);
+                jp = i;
+            }
+        ipiv[j] = jp;
+        if (a[jp + 3 * j] != 0.0) {
+            if (jp != j)
+                for (int k = 0; k < n; k++) {
+                    double t = a[j + 3 * k];
+                    a[j + 3 * k] = a[jp + 3 * k];
+                    a[jp + 3 * k] = t;
+                }
+            if (j < n - 1) {
+                if (fabs(a[j + 3 * j]) >= sfmin) {
+                    double r = 1.0 / a[j + 3 * j];
+                    for (int i = j + 1; i < n; i++)
+                        a[i + 3 * j] *= r;
+                } else {
+                    for (int i = j + 1; i < n; i++)
+                        a[i + 3 * j] = a[i + 3 * j] / a[j + 3 * j];
+                }
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        if (j < n - 1)
+            for (int k = j + 1; k < n; k++) {
+                if (a[j + 3 * k] != 0.0) {
+                    double temp = -a[j + 3 * k];
+                    for (int i = j + 1; i < n; i++)
+                        a[i + 3 * k] = a[i + 3 * k] + a[i + 3 * j] * temp;
+                }
+            }
+    }
+    if (info != 0)
+        return info;
+    for (int i = 0; i < n; i++)
+        if (ipiv[i] != i) {
+            double t = b[i];
+            b[i] = b[ipiv[i]];
+            b[ipiv[i]] = t;
+        }
+    for (int k = 0; k < n; k++)
+        if (b[k] != 0.0)
+            for (int i = k + 1; i < n; i++)
+                b[i] = b[i] - b[k] * a[i + 3 * k];
+    for (int k = n - 1; k >= 0; k--)
+        if (b[k] != 0.0) {
+            b[k] = b[k] / a[k + 3 * k];
+            for (int i = 0; i < k; i++)
+                b[i] = b[i] - b[k] * a[i + 3 * k];
+        }
+    return 0;
+}
+
+/* SkyMap::solve_map (skymap.cpp:134-224): per pixel, hits = int(AtA[0]);
+ * with at least 3 hits the 3x3 system AtA x = AtD is solved as DGESV does
+ * (the matrix assembled column-major as at skymap.cpp:159-172); a singular
+ * system leaves I, Q, U = AtD and sets hits to 0; fewer than 3 hits give
+ * I = Q = U = 0 (INSUFFCOV) and keep the hits.  Returns the number of
+ * singular pixels (the reference prints each one). */
+long or_skymap_solve(int nside, const double *AtA, const double *AtD, float *I, float *Q,
+                     float *U, int *hits)
+{
+    long npix = 12L * nside * nside;
+    long singular = 0;
+    for (long p = 0; p < npix; p++) {
+        const double *m = AtA + 9 * p;
+        double a[9] = {m[0], m[3], m[6], m[1], m[4], m[7], m[2], m[5], m[8]};
+        double b[3] = {AtD[3 * p + 0], AtD[3 * p + 1], AtD[3 * p + 2]};
+        double ii = 0.0, qq = 0.0, uu = 0.0;
+        int h = (int)m[0];
+        if (h >= 3) {
+            int info = o_dgesv3(a, b);
+            ii = b[0];
+            qq = b[1];
+            uu = b[2];
+            if (info != 0) {
+                h = 0;
+                singular++;
+            }
+        }
+        I[p] = (float)ii;
+        Q[p] = (float)qq;
+        U[p] = (float)uu;
+        hits[p] = h;
+    }
+    return singular;
+}
+
+/* The 3x3 solve alone, for pinning against LAPACK: n systems, a column-major
+ * [n][9], b [n][3] overwritten; info [n]. */
+void or_dgesv3_many(long n, double *a, double *b, int *info)
+{
+    for (long i = 0; i < n; i++)
+        info[i] = o_dgesv3(a + 9 * i, b + 3 * i);
+}

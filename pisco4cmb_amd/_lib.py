@@ -73,6 +73,17 @@ SIGNATURES = {
                                           P, P, P, P]),
     "pisco_dev_batched_adjoint_map": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_long,
                                               P, P, P, P, P, P, P]),
+    "pisco_skymap_create": (c_int, [c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "pisco_skymap_destroy": (None, [c_void_p]),
+    "pisco_skymap_reset": (c_int, [c_void_p]),
+    "pisco_skymap_set_chunk": (c_int, [c_void_p, c_long]),
+    "pisco_skymap_accumulate": (c_int, [c_void_p, c_long, P, P, P, c_int, P, P]),
+    "pisco_dev_skymap_accumulate": (c_int, [c_void_p, c_void_p, c_long, P, P, P, c_int, c_int,
+                                            P, P]),
+    "pisco_skymap_skipped": (c_int, [c_void_p, ctypes.POINTER(c_long)]),
+    "pisco_skymap_solve": (c_int, [c_void_p, P, P, P, P, ctypes.POINTER(c_long)]),
+    "pisco_dev_skymap_solve": (c_int, [c_void_p, c_void_p, P, P, P, P, ctypes.POINTER(c_long)]),
+    "pisco_skymap_get_normal": (c_int, [c_void_p, P, P]),
     "pisco_dev_pix2ang": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_ang2pix": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_get_interpol": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P, P]),

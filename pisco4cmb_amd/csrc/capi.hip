@@ -1083,3 +1083,292 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     HIPCHK(launch_f64_to_f32(bp.gU, gU, npix, st));
     return PISCO_OK;
 }
+
+/* ======================================================================== */
+/* Binned map-maker (SkyMap)                                                */
+/* ======================================================================== */
+
+struct pisco_skymap {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nside = 0;
+    long npix = 0;
+    long chunk = 1L << 24;               /* samples per accumulation pass */
+    DevBuf<double> acc;                  /* MM_PLANES x npix */
+    DevBuf<int> count, cur, off, tsum;   /* per pixel / per scan tile */
+    DevBuf<int> pix, idx;                /* per sample of a pass */
+    DevBuf<unsigned long long> counters; /* [0] skipped, [1] singular */
+    DevBuf<float> stage;                 /* host-API inputs */
+    DevBuf<float> out;                   /* host-API outputs */
+    DevBuf<int> hits;
+};
+
+namespace {
+
+int map_device(pisco_skymap *m)
+{
+    HIPCHK(hipSetDevice(m->device));
+    return PISCO_OK;
+}
+
+int map_accumulate(pisco_skymap *m, hipStream_t st, long nsamples, const float *ra,
+                   const float *dec, const float *pa, long ptg_stride, int ndet,
+                   const float *pol, const float *data)
+{
+    const long total = nsamples * (long)ndet;
+    const long chunk = std::min<long>(m->chunk, total);
+    CHK(m->pix.ensure(chunk));
+    CHK(m->idx.ensure(chunk));
+    MapScratch s;
+    s.pix = m->pix.p;
+    s.idx = m->idx.p;
+    s.count = m->count.p;
+    s.cur = m->cur.p;
+    s.off = m->off.p;
+    s.tsum = m->tsum.p;
+    s.skipped = m->counters.p;
+    Hpx h = hpx_make(m->nside);
+    for (long t0 = 0; t0 < total; t0 += chunk) {
+        MapChunk c;
+        c.ra = ra;
+        c.dec = dec;
+        c.pa = pa;
+        c.pol = pol;
+        c.data = data;
+        c.nsamples = nsamples;
+        c.ptg_stride = ptg_stride;
+        c.t0 = t0;
+        c.len = (int)std::min(chunk, total - t0);
+        hipError_t e = launch_map_accumulate(h, c, s, m->acc.p, st);
+        if (e != hipSuccess)
+            return fail(PISCO_ERR_DEVICE, "map accumulation launch failed: %s",
+                        hipGetErrorString(e));
+    }
+    return PISCO_OK;
+}
+
+int map_check(pisco_skymap *m, long nsamples, int ndet)
+{
+    if (!m)
+        return fail(PISCO_ERR_INVALID, "NULL map");
+    if (nsamples < 0 || ndet < 0)
+        return fail(PISCO_ERR_INVALID, "negative nsamples or ndet");
+    return PISCO_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pisco_skymap_create(int device, int nside, pisco_skymap **out)
+{
+    if (!out)
+        return fail(PISCO_ERR_INVALID, "NULL out");
+    *out = nullptr;
+    if (!valid_nside(nside))
+        return fail(PISCO_ERR_INVALID, "nside %d out of range [1, %d]", nside, kMaxNside);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(PISCO_ERR_INVALID, "device %d not present (%d devices)", device, ndev);
+    pisco_skymap *m = new pisco_skymap;
+    m->device = device;
+    m->nside = nside;
+    m->npix = 12L * nside * nside;
+    int rc = map_device(m);
+    if (rc == PISCO_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(PISCO_ERR_DEVICE, "hipStreamCreate failed");
+    if (rc == PISCO_OK)
+        rc = m->acc.ensure(MM_PLANES * m->npix);
+    if (rc == PISCO_OK)
+        rc = m->count.ensure(m->npix);
+    if (rc == PISCO_OK)
+        rc = m->cur.ensure(m->npix);
+    if (rc == PISCO_OK)
+        rc = m->off.ensure(m->npix + 1);
+    if (rc == PISCO_OK)
+        rc = m->tsum.ensure(map_tiles((int)m->npix));
+    if (rc == PISCO_OK)
+        rc = m->counters.ensure(2);
+    if (rc == PISCO_OK)
+        rc = pisco_skymap_reset(m);
+    if (rc != PISCO_OK) {
+        std::string keep = g_err;
+        pisco_skymap_destroy(m);
+        g_err = keep;
+        return rc;
+    }
+    *out = m;
+    return PISCO_OK;
+}
+
+void pisco_skymap_destroy(pisco_skymap *m)
+{
+    if (!m)
+        return;
+    (void)hipSetDevice(m->device);
+    if (m->stream)
+        (void)hipStreamSynchronize(m->stream);
+    m->acc.release();
+    m->count.release();
+    m->cur.release();
+    m->off.release();
+    m->tsum.release();
+    m->pix.release();
+    m->idx.release();
+    m->counters.release();
+    m->stage.release();
+    m->out.release();
+    m->hits.release();
+    if (m->stream)
+        (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int pisco_skymap_reset(pisco_skymap *m)
+{
+    if (!m)
+        return fail(PISCO_ERR_INVALID, "NULL map");
+    CHK(map_device(m));
+    HIPCHK(launch_zero_f64(m->acc.p, MM_PLANES * m->npix, m->stream));
+    HIPCHK(launch_fill_u64(m->counters.p, 2, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return PISCO_OK;
+}
+
+int pisco_skymap_set_chunk(pisco_skymap *m, long samples)
+{
+    if (!m)
+        return fail(PISCO_ERR_INVALID, "NULL map");
+    if (samples < 1 || samples > (1L << 30))
+        return fail(PISCO_ERR_INVALID, "chunk must be in [1, 2^30] samples");
+    m->chunk = samples;
+    return PISCO_OK;
+}
+
+int pisco_dev_skymap_accumulate(pisco_skymap *m, void *stream, long nsamples, const float *ra,
+                                const float *dec, const float *pa, int shared_pointing,
+                                int ndet, const float *pol, const float *data)
+{
+    CHK(map_check(m, nsamples, ndet));
+    if (nsamples == 0 || ndet == 0)
+        return PISCO_OK;
+    if (!ra || !dec || !pa || !pol || !data)
+        return fail(PISCO_ERR_INVALID, "NULL input array");
+    CHK(map_device(m));
+    return map_accumulate(m, (hipStream_t)stream, nsamples, ra, dec, pa,
+                          shared_pointing ? 0 : nsamples, ndet, pol, data);
+}
+
+int pisco_skymap_accumulate(pisco_skymap *m, long nsamples, const float *ra, const float *dec,
+                            const float *pa, int ndet, const float *pol, const float *data)
+{
+    CHK(map_check(m, nsamples, ndet));
+    if (nsamples == 0 || ndet == 0)
+        return PISCO_OK;
+    if (!ra || !dec || !pa || !pol || !data)
+        return fail(PISCO_ERR_INVALID, "NULL input array");
+    CHK(map_device(m));
+    const long n = nsamples * (long)ndet;
+    CHK(m->stage.ensure(4 * n + ndet));
+    float *d = m->stage.p;
+    unsigned long long before = 0, after = 0;
+    HIPCHK(hipMemcpyAsync(&before, m->counters.p, sizeof before, hipMemcpyDeviceToHost,
+                          m->stream));
+    HIPCHK(hipMemcpyAsync(d, ra, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(d + n, dec, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(d + 2 * n, pa, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(d + 3 * n, data, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(d + 4 * n, pol, ndet * sizeof(float), hipMemcpyHostToDevice,
+                          m->stream));
+    CHK(map_accumulate(m, m->stream, nsamples, d, d + n, d + 2 * n, nsamples, ndet, d + 4 * n,
+                       d + 3 * n));
+    HIPCHK(hipMemcpyAsync(&after, m->counters.p, sizeof after, hipMemcpyDeviceToHost,
+                          m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (after != before)
+        return fail(PISCO_ERR_INVALID,
+                    "%llu samples have theta = pi/2 - dec outside [0, pi] or a non-finite ra; "
+                    "they were skipped",
+                    after - before);
+    return PISCO_OK;
+}
+
+int pisco_skymap_skipped(pisco_skymap *m, long *count)
+{
+    if (!m || !count)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    CHK(map_device(m));
+    unsigned long long v = 0;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(&v, m->counters.p, sizeof v, hipMemcpyDeviceToHost));
+    *count = (long)v;
+    return PISCO_OK;
+}
+
+int pisco_dev_skymap_solve(pisco_skymap *m, void *stream, float *I, float *Q, float *U,
+                           int *hits, long *singular)
+{
+    if (!m || !I || !Q || !U || !hits)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    CHK(map_device(m));
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(launch_fill_u64(m->counters.p + 1, 1, st));
+    HIPCHK(launch_map_solve((int)m->npix, m->acc.p, I, Q, U, hits, m->counters.p + 1, st));
+    if (singular) {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, m->counters.p + 1, sizeof v, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        *singular = (long)v;
+    }
+    return PISCO_OK;
+}
+
+int pisco_skymap_solve(pisco_skymap *m, float *I, float *Q, float *U, int *hits,
+                       long *singular)
+{
+    if (!m || !I || !Q || !U || !hits)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    CHK(map_device(m));
+    const long n = m->npix;
+    CHK(m->out.ensure(3 * n));
+    CHK(m->hits.ensure(n));
+    CHK(pisco_dev_skymap_solve(m, m->stream, m->out.p, m->out.p + n, m->out.p + 2 * n, m->hits.p,
+                               singular));
+    HIPCHK(hipMemcpyAsync(I, m->out.p, n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(Q, m->out.p + n, n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(U, m->out.p + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost,
+                          m->stream));
+    HIPCHK(hipMemcpyAsync(hits, m->hits.p, n * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return PISCO_OK;
+}
+
+int pisco_skymap_get_normal(pisco_skymap *m, double *AtA, double *AtD)
+{
+    if (!m || !AtA || !AtD)
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    CHK(map_device(m));
+    const long n = m->npix;
+    std::vector<double> h((size_t)(MM_PLANES * n));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(h.data(), m->acc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double *P[MM_PLANES];
+    for (int k = 0; k < MM_PLANES; k++)
+        P[k] = h.data() + k * n;
+    for (long p = 0; p < n; p++) {
+        double *a = AtA + 9 * p;
+        a[0] = P[MM_N][p];
+        a[1] = a[3] = P[MM_C][p];
+        a[2] = a[6] = P[MM_S][p];
+        a[4] = P[MM_CC][p];
+        a[5] = a[7] = P[MM_CS][p];
+        a[8] = P[MM_SS][p];
+        AtD[3 * p + 0] = P[MM_D][p];
+        AtD[3 * p + 1] = P[MM_CD][p];
+        AtD[3 * p + 2] = P[MM_SD][p];
+    }
+    return PISCO_OK;
+}
+
+} /* extern "C" */

@@ -75,6 +75,34 @@ struct BatchParams {
     double *gI, *gQ, *gU;
 };
 
+/* Binned map-maker.  Samples of a chunk are global indices t in
+ * [t0, t0 + len) of the detector-major arrays (t = det * nsamples + s);
+ * pointing of sample t is at det * ptg_stride + s (ptg_stride 0: every
+ * detector shares one pointing). */
+struct MapChunk {
+    const float *ra, *dec, *pa;
+    const float *pol;         /* [ndet] detector polarisation angles */
+    const float *data;        /* [ndet][nsamples] */
+    long nsamples, ptg_stride, t0;
+    int len;
+};
+/* FP64 planes of the normal equations, npix each: hits, sum c, sum s,
+ * sum cc, sum cs, sum ss (AtA) and sum d, sum cd, sum sd (AtD) */
+enum { MM_N, MM_C, MM_S, MM_CC, MM_CS, MM_SS, MM_D, MM_CD, MM_SD, MM_PLANES };
+struct MapScratch {
+    int *pix, *idx;           /* [chunk] */
+    int *count, *cur;         /* [npix] */
+    int *off;                 /* [npix + 1] */
+    int *tsum;                /* [map_tiles(npix)] */
+    unsigned long long *skipped;
+};
+int map_tiles(int npix);
+hipError_t launch_map_accumulate(const Hpx &h, const MapChunk &c, const MapScratch &s,
+                                 double *acc, hipStream_t st);
+hipError_t launch_map_solve(int npix, const double *acc, float *I, float *Q, float *U,
+                            int *hits, unsigned long long *singular, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long *x, long n, hipStream_t st);
+
 hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st);
 
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);

@@ -27,7 +27,7 @@ import numpy as np
 from . import _lib
 from ._lib import RunConfig, check, lib, ptr_array
 
-__all__ = ["RunConfig", "Sky", "Scan", "PolBeam", "GPUConvolver", "Convolver",
+__all__ = ["RunConfig", "Sky", "Scan", "PolBeam", "GPUConvolver", "Convolver", "SkyMap",
            "default_run_config", "beam_npix_for_radius", "ang2pix", "pix2ang"]
 
 
@@ -307,3 +307,93 @@ class Convolver:
             raise ValueError("valid polFlags are 'a', 'b' and 'p'")
         self._gpu.exec_convolution(data_a if polFlag != "b" else None,
                                    data_b if polFlag != "a" else None, polFlag)
+
+
+class SkyMap:
+    """SkyMap(nside) (src/SkyMap/skymap.hpp:13-55): binned I, Q, U map-maker,
+    its normal equations held on a GPU.
+
+        m = SkyMap(nside)
+        m.accumulate_data(nsamples, ra, dec, pa, ndetectors, detPolAngles, data)
+        m.solve_map()
+        I, Q, U, hits = m.get_stokes_I(), m.get_stokes_Q(), m.get_stokes_U(), m.get_hitmap()
+
+    ra, dec, pa and data hold ndetectors * nsamples float32 values,
+    detector-major (skymap.cpp:91-97).  The accumulators start at zero.  A
+    sample with theta = pi/2 - dec outside [0, pi] (where the reference's
+    Healpix_Base::ang2pix throws) raises ValueError after the call has
+    accumulated every other sample."""
+
+    INSUFFCOV = 0
+
+    def __init__(self, nside, device=0):
+        self.nside = int(nside)
+        self.nPixels = 12 * self.nside * self.nside
+        h = ctypes.c_void_p()
+        check(lib.pisco_skymap_create(int(device), self.nside, ctypes.byref(h)))
+        self.handle = h
+        self.stokesI = np.zeros(self.nPixels, np.float32)
+        self.stokesQ = np.zeros(self.nPixels, np.float32)
+        self.stokesU = np.zeros(self.nPixels, np.float32)
+        self.hitsMap = np.zeros(self.nPixels, np.int32)
+        self.singular = 0
+
+    def get_nside(self):
+        return self.nside
+
+    def size(self):
+        return self.nPixels
+
+    def accumulate_data(self, nsamples, ra, dec, pa, ndetectors, detPolAngles, data):
+        n = int(nsamples) * int(ndetectors)
+        ra, dec, pa, data = (_f32(a, n, nm) for a, nm in ((ra, "ra"), (dec, "dec"), (pa, "pa"),
+                                                          (data, "data")))
+        pol = _f32(detPolAngles, int(ndetectors), "detPolAngles")
+        rc = lib.pisco_skymap_accumulate(self.handle, int(nsamples), _addr(ra), _addr(dec),
+                                         _addr(pa), int(ndetectors), _addr(pol), _addr(data))
+        if rc == _lib.PISCO_ERR_INVALID:
+            raise ValueError(lib.pisco_last_error().decode())
+        check(rc)
+
+    def solve_map(self):
+        sing = ctypes.c_long()
+        check(lib.pisco_skymap_solve(self.handle, _addr(self.stokesI), _addr(self.stokesQ),
+                                     _addr(self.stokesU), _addr(self.hitsMap),
+                                     ctypes.byref(sing)))
+        self.singular = sing.value
+
+    def get_stokes_I(self):
+        return self.stokesI
+
+    def get_stokes_Q(self):
+        return self.stokesQ
+
+    def get_stokes_U(self):
+        return self.stokesU
+
+    def get_hitmap(self):
+        return self.hitsMap
+
+    def get_normal(self):
+        """(AtA [npix, 9], AtD [npix, 3]) as the reference stores them."""
+        AtA = np.empty((self.nPixels, 9))
+        AtD = np.empty((self.nPixels, 3))
+        check(lib.pisco_skymap_get_normal(self.handle, _addr(AtA), _addr(AtD)))
+        return AtA, AtD
+
+    def reset(self):
+        check(lib.pisco_skymap_reset(self.handle))
+
+    def set_chunk(self, samples):
+        check(lib.pisco_skymap_set_chunk(self.handle, int(samples)))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None:
+            lib.pisco_skymap_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -57,6 +57,9 @@ class Oracle:
             "or_vec_cos": (None, [c_long, P, P]),
             "or_vec_acos": (None, [c_long, P, P]),
             "or_vec_atan2": (None, [c_long, P, P, P]),
+            "or_skymap_accumulate": (c_long, [c_int, c_long, P, P, P, c_long, c_int, P, P, P, P]),
+            "or_skymap_solve": (c_long, [c_int, P, P, P, P, P, P]),
+            "or_dgesv3_many": (None, [c_long, P, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -189,6 +192,41 @@ class Oracle:
         sizes = np.empty(ra.size, np.int32)
         total = self.L.or_disc_sizes(nside_sky, radius, ra.size, _a(ra), _a(dec), _a(sizes))
         return total, sizes
+
+    # --- SkyMap
+    def skymap_accumulate(self, nside, ra, dec, pa, pol_angles, data, AtA=None, AtD=None,
+                          shared_pointing=False):
+        """ra, dec, pa: [ndet, n] (or [n] with shared_pointing); data [ndet, n].
+        Returns (AtA [npix, 9], AtD [npix, 3], skipped)."""
+        data = np.ascontiguousarray(data, np.float32)
+        ndet, n = data.shape
+        ra, dec, pa = (np.ascontiguousarray(x, np.float32) for x in (ra, dec, pa))
+        pol = np.ascontiguousarray(pol_angles, np.float32)
+        npix = 12 * nside * nside
+        AtA = np.zeros((npix, 9)) if AtA is None else AtA
+        AtD = np.zeros((npix, 3)) if AtD is None else AtD
+        skipped = self.L.or_skymap_accumulate(nside, n, _a(ra), _a(dec), _a(pa),
+                                              0 if shared_pointing else n, ndet, _a(pol),
+                                              _a(data), _a(AtA), _a(AtD))
+        return AtA, AtD, skipped
+
+    def skymap_solve(self, nside, AtA, AtD):
+        npix = 12 * nside * nside
+        AtA = np.ascontiguousarray(AtA, np.float64)
+        AtD = np.ascontiguousarray(AtD, np.float64)
+        I, Q, U = (np.empty(npix, np.float32) for _ in range(3))
+        hits = np.empty(npix, np.int32)
+        singular = self.L.or_skymap_solve(nside, _a(AtA), _a(AtD), _a(I), _a(Q), _a(U),
+                                          _a(hits))
+        return I, Q, U, hits, singular
+
+    def dgesv3(self, a, b):
+        """a: [n, 3, 3] (row i, column j), b: [n, 3] -> (x, info)."""
+        a = np.ascontiguousarray(np.swapaxes(np.asarray(a, np.float64), 1, 2))  # column-major
+        b = np.array(b, np.float64, order="C", copy=True)
+        info = np.empty(a.shape[0], np.int32)
+        self.L.or_dgesv3_many(a.shape[0], _a(a), _a(b), _a(info))
+        return b, info
 
     def vec(self, name, x, y=None):
         x = np.ascontiguousarray(x, np.float64)
