@@ -21,12 +21,14 @@ with |D_i| the exact disc sizes (device query_disc).  achieved = bytes of
 the dominant kernel / its mean duration (HIP events on the launch stream).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
-                  [--no-cpu] [--no-forms] [--no-batched] [--calib-copy-mib M]
+                  [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker]
+                  [--calib-copy-mib M]
 
 Also reported: the plain drop-in forward of the NSIDE 256 / 1.5 deg workload
 (device-resident and through the host API including PCIe copies), the
 batched 64-detector forward + adjoint against the plain per-pair launches,
-and the CPU oracle on a prefix of the headline workload.
+the binned map-maker on 64 detectors, and the CPU oracle on a prefix of the
+headline workload.
 """
 import argparse
 import ctypes
@@ -71,6 +73,8 @@ def parse():
                    help="skip the plain drop-in forward of the NSIDE 256 workload")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched 64-detector workload")
+    p.add_argument("--no-mapmaker", action="store_true",
+                   help="skip the binned map-maker workload")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="run one device-to-device copy of this size first (a kernel of "
                         "known traffic, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -208,6 +212,8 @@ def main():
         out["plain_forward_nside256"] = plain_forward(ctx, stream, dev)
     if not args.no_batched:
         out["batched_64det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky)
+    if not args.no_mapmaker:
+        out["map_maker_64det"] = map_maker_workload(dev, nside_sky, with_cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra,
                                            dec, pa, g)
@@ -373,6 +379,75 @@ def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=
                               "hbm_frac": b_fa / (t_pf + t_pa) / HBM_PEAK},
             "batched_fwd_adj_vs_plain_forward": t_pf / t_step,
             "batched_fwd_adj_vs_plain_fwd_adj": (t_pf + t_pa) / t_step}
+
+
+def map_maker_workload(dev, nside_sky, n=250_000, ndet=64, reps=3, with_cpu=True,
+                       cpu_dets=8):
+    """Binned map-making (SkyMap::accumulate_data + solve_map) of 64 detectors
+    x 2.5e5 samples sharing one CES pointing, each detector with its own
+    polarisation angle, into a sky of NSIDE 512.  Gather model: each
+    detector-sample reads its pointing and datum (16 B); each touched pixel
+    reads and writes its nine FP64 sums once per call (144 B); the solve
+    reads every pixel's hit count and writes I, Q, U, hits (24 B per pixel)
+    and reads the other eight sums of pixels with 3 or more hits (64 B).  The CPU
+    oracle (sequential, as the reference's loops) runs on the first
+    `cpu_dets` detectors."""
+    import numpy as np
+    import torch
+
+    from pisco4cmb_amd import inputs
+    from pisco4cmb_amd.mapmaker import DeviceSkyMap
+    ra, dec, pa = inputs.ces_scan(n, t0=3600.0)
+    rng = np.random.default_rng(65)
+    pol = rng.uniform(-np.pi, np.pi, ndet).astype(np.float32)
+    data = rng.normal(0, 30, (ndet, n)).astype(np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rat, dect, pat, polt, datat = t(ra), t(dec), t(pa), t(pol), t(data)
+    m = DeviceSkyMap(nside_sky, dev)
+
+    def tm(f):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_acc = tm(lambda: m.accumulate(rat, dect, pat, polt, datat, shared_pointing=True))
+    t_sol = tm(m.solve)
+    hits = m.solve()[3]
+    touched = int((hits > 0).sum().item())
+    solved = int((hits >= 3).sum().item())
+    m.close()
+    npix = 12 * nside_sky ** 2
+    b_acc = ndet * n * 16 + touched * 144
+    b_sol = npix * 24 + solved * 64
+    res = {"workload": "map-making of %d detectors x %d samples (one CES pointing, per-detector "
+                       "polarisation angles) into NSIDE %d" % (ndet, n, nside_sky),
+           "touched_pixels": touched, "solved_pixels": solved,
+           "accumulate": {"det_samples_per_s": ndet * n / t_acc, "s": t_acc,
+                          "hbm_frac": b_acc / t_acc / HBM_PEAK},
+           "solve": {"pixels_per_s": npix / t_sol, "s": t_sol,
+                     "hbm_frac": b_sol / t_sol / HBM_PEAK}}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind
+        o = oracle_bind.Oracle()
+        c0 = time.time()
+        AtA, AtD, _ = o.skymap_accumulate(nside_sky, ra, dec, pa, pol[:cpu_dets],
+                                          data[:cpu_dets], shared_pointing=True)
+        c1 = time.time()
+        o.skymap_solve(nside_sky, AtA, AtD)
+        c2 = time.time()
+        res["cpu_oracle"] = {"accumulate_det_samples_per_s": cpu_dets * n / (c1 - c0),
+                             "solve_pixels_per_s": npix / (c2 - c1), "cores": 1,
+                             "sample": "first %d detectors, sequential as skymap.cpp"
+                                       % cpu_dets}
+    return res
 
 
 def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g):

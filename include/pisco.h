@@ -211,7 +211,7 @@ void pisco_skymap_destroy(pisco_skymap *map);
 /* Zero the accumulators (a fresh SkyMap). */
 int pisco_skymap_reset(pisco_skymap *map);
 /* Samples per accumulation pass (default 2^24; the sums do not depend on
- * it).  Scratch is 8 bytes per sample of a pass. */
+ * it).  Scratch is about 41 bytes per sample of a pass. */
 int pisco_skymap_set_chunk(pisco_skymap *map, long samples);
 
 /* SkyMap::accumulate_data (skymap.cpp:68-132), host buffers.  ra, dec, pa

@@ -1095,8 +1095,13 @@ struct pisco_skymap {
     long npix = 0;
     long chunk = 1L << 24;               /* samples per accumulation pass */
     DevBuf<double> acc;                  /* MM_PLANES x npix */
-    DevBuf<int> count, cur, off, tsum;   /* per pixel / per scan tile */
-    DevBuf<int> pix, idx;                /* per sample of a pass */
+    DevBuf<int> count, off, tsum;        /* per pixel / per scan tile */
+    DevBuf<unsigned> keys;               /* 3 x chunk: pixels and the sort's buffers */
+    DevBuf<int> vals;                    /* 2 x chunk */
+    DevBuf<double> terms;                /* 2 x chunk: cos 2psi, sin 2psi */
+    DevBuf<float> tdata;                 /* chunk */
+    DevBuf<int> hist;                    /* 256 x tiles, its scan (+1), scan tiles */
+    DevBuf<int> ppix;                    /* pixels of a shared pointing */
     DevBuf<unsigned long long> counters; /* [0] skipped, [1] singular */
     DevBuf<float> stage;                 /* host-API inputs */
     DevBuf<float> out;                   /* host-API outputs */
@@ -1117,17 +1122,34 @@ int map_accumulate(pisco_skymap *m, hipStream_t st, long nsamples, const float *
 {
     const long total = nsamples * (long)ndet;
     const long chunk = std::min<long>(m->chunk, total);
-    CHK(m->pix.ensure(chunk));
-    CHK(m->idx.ensure(chunk));
+    const long nh = 256L * map_tiles((int)chunk);
+    CHK(m->keys.ensure(3 * chunk));
+    CHK(m->vals.ensure(2 * chunk));
+    CHK(m->terms.ensure(2 * chunk));
+    CHK(m->tdata.ensure(chunk));
+    CHK(m->hist.ensure(2 * nh + 1 + map_tiles((int)nh)));
     MapScratch s;
-    s.pix = m->pix.p;
-    s.idx = m->idx.p;
+    for (int k = 0; k < 3; k++)
+        s.keys[k] = m->keys.p + k * chunk;
+    s.vals[0] = nullptr;
+    s.vals[1] = m->vals.p;
+    s.vals[2] = m->vals.p + chunk;
+    s.tcs = (double2 *)m->terms.p;
+    s.td = m->tdata.p;
     s.count = m->count.p;
-    s.cur = m->cur.p;
     s.off = m->off.p;
     s.tsum = m->tsum.p;
+    s.hist = m->hist.p;
+    s.hoff = m->hist.p + nh;
+    s.htsum = m->hist.p + 2 * nh + 1;
     s.skipped = m->counters.p;
+    s.ppix = nullptr;
     Hpx h = hpx_make(m->nside);
+    if (ptg_stride == 0 && ndet > 1) {
+        CHK(m->ppix.ensure(nsamples));
+        HIPCHK(launch_map_ptg_pixel(h, ra, dec, nsamples, m->ppix.p, st));
+        s.ppix = m->ppix.p;
+    }
     for (long t0 = 0; t0 < total; t0 += chunk) {
         MapChunk c;
         c.ra = ra;
@@ -1183,8 +1205,6 @@ int pisco_skymap_create(int device, int nside, pisco_skymap **out)
     if (rc == PISCO_OK)
         rc = m->count.ensure(m->npix);
     if (rc == PISCO_OK)
-        rc = m->cur.ensure(m->npix);
-    if (rc == PISCO_OK)
         rc = m->off.ensure(m->npix + 1);
     if (rc == PISCO_OK)
         rc = m->tsum.ensure(map_tiles((int)m->npix));
@@ -1211,11 +1231,14 @@ void pisco_skymap_destroy(pisco_skymap *m)
         (void)hipStreamSynchronize(m->stream);
     m->acc.release();
     m->count.release();
-    m->cur.release();
     m->off.release();
     m->tsum.release();
-    m->pix.release();
-    m->idx.release();
+    m->keys.release();
+    m->vals.release();
+    m->terms.release();
+    m->tdata.release();
+    m->hist.release();
+    m->ppix.release();
     m->counters.release();
     m->stage.release();
     m->out.release();

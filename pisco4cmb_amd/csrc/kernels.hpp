@@ -90,15 +90,28 @@ struct MapChunk {
  * sum cc, sum cs, sum ss (AtA) and sum d, sum cd, sum sd (AtD) */
 enum { MM_N, MM_C, MM_S, MM_CC, MM_CS, MM_SS, MM_D, MM_CD, MM_SD, MM_PLANES };
 struct MapScratch {
-    int *pix, *idx;           /* [chunk] */
-    int *count, *cur;         /* [npix] */
+    unsigned *keys[3];        /* [chunk] each: pixels (0) and the sort's two buffers */
+    int *vals[3];             /* [chunk] (vals[0] unused) */
+    double2 *tcs;             /* [chunk] (cos 2psi, sin 2psi) in sorted order */
+    float *td;                /* [chunk] data in sorted order */
+    int *count;               /* [npix] */
     int *off;                 /* [npix + 1] */
     int *tsum;                /* [map_tiles(npix)] */
+    int *hist, *hoff;         /* [256 * map_tiles(chunk)] (+1 for hoff) */
+    int *htsum;               /* [map_tiles(256 * map_tiles(chunk))] */
+    const int *ppix;          /* [nsamples] pixels of a shared pointing, or NULL */
     unsigned long long *skipped;
 };
-int map_tiles(int npix);
+/* scan tiles of an array of n counts (4096 per tile) */
+int map_tiles(int n);
+/* radix passes of 8 bits that order the pixels of a map of npix pixels */
+int map_sort_passes(int npix);
 hipError_t launch_map_accumulate(const Hpx &h, const MapChunk &c, const MapScratch &s,
                                  double *acc, hipStream_t st);
+/* ppix[s] = pixel of pointing s (-1: skipped), for a pointing shared by
+ * every detector */
+hipError_t launch_map_ptg_pixel(const Hpx &h, const float *ra, const float *dec, long n,
+                                int *ppix, hipStream_t st);
 hipError_t launch_map_solve(int npix, const double *acc, float *I, float *Q, float *U,
                             int *hits, unsigned long long *singular, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *x, long n, hipStream_t st);
