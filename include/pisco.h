@@ -197,6 +197,22 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *ctx, void *stream, const pisco_batc
                                   const float *dec, const float *pa, const float *g,
                                   float *gI, float *gQ, float *gU);
 
+/* The batched forward with a pointing per pair: ra, dec and pa hold
+ * npairs x nsamples floats, pair k (in the caller's order) at k * nsamples,
+ * e.g. the float outputs of pisco_dev_focal_plane_to_sky for each pair's
+ * focal-plane position.  pa_offset is still added.  One launch over
+ * (pair, sample) items; pairs do not share visit geometry. */
+int pisco_dev_batched_forward_pp(pisco_ctx *ctx, void *stream, const pisco_batch *batch,
+                                 int nside_sky, const float *I, const float *Q,
+                                 const float *U, long nsamples, const float *ra,
+                                 const float *dec, const float *pa, float *tod);
+/* Sky adjoint of pisco_dev_batched_forward_pp: one LDS-windowed sky adjoint
+ * per pair, adding into one FP64 gradient. */
+int pisco_dev_batched_adjoint_map_pp(pisco_ctx *ctx, void *stream, const pisco_batch *batch,
+                                     int nside_sky, long nsamples, const float *ra,
+                                     const float *dec, const float *pa, const float *g,
+                                     float *gI, float *gQ, float *gU);
+
 /* ---- binned map-maker (SkyMap) ------------------------------------------ */
 
 /* The normal equations AtA (3x3) and AtD (3) of every pixel of a RING map,
@@ -253,6 +269,31 @@ int pisco_dev_skymap_solve(pisco_skymap *map, void *stream, float *I, float *Q, 
 /* The sums in the reference's storage (skymap.hpp:42-43): AtA [npix][9]
  * row-major 3x3, AtD [npix][3], host arrays. */
 int pisco_skymap_get_normal(pisco_skymap *map, double *AtA, double *AtD);
+
+/* ---- detector pointing (BPoint::focal_plane_to_sky) ---------------------- */
+
+/* BPoint::focal_plane_to_sky (src/Bpoint/bpoint.cpp:553-674) on the device:
+ * detector offsets det_dx (right), det_dy (up) in the receiver frame seen
+ * looking out, and polarisation angles det_pol_angle (ccw from the receiver
+ * y axis), all [ndets] radians; array-centre coordinates x_ac, y_ac and
+ * angle rot_ac [nsamples] radians.  flag 0: az/el (x_ac azimuth, rot_ac
+ * boresight angle ccw from zenith), 1: ra/dec (rot_ac position angle ccw from
+ * the pole), 2: source-relative.  Outputs [ndets][nsamples] detector-major,
+ * each NULL or device memory: x_det, y_det, rot_det in FP64 and, rounded to
+ * float, x32, y32, rot32 (with flag 1 these are a Scan's ra, dec, pa).  All
+ * input pointers are device memory; FP64 throughout, the same bits as the
+ * CPU restatement. */
+int pisco_dev_focal_plane_to_sky(pisco_ctx *ctx, void *stream, int ndets, const double *det_dx,
+                                 const double *det_dy, const double *det_pol_angle,
+                                 long nsamples, const double *x_ac, const double *y_ac,
+                                 const double *rot_ac, int flag, double *x_det, double *y_det,
+                                 double *rot_det, float *x32, float *y32, float *rot32);
+/* The same with host arrays (the reference's signature, BPoint.h:99-103;
+ * outputs of ndets * nsamples doubles each). */
+int pisco_focal_plane_to_sky(pisco_ctx *ctx, int ndets, const double *det_dx,
+                             const double *det_dy, const double *det_pol_angle, long nsamples,
+                             const double *x_ac, const double *y_ac, const double *rot_ac,
+                             int flag, double *x_det, double *y_det, double *rot_det);
 
 /* ---- HEALPix / matching primitives on the device (for verification) ---- */
 

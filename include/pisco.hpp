@@ -8,6 +8,8 @@
  *   PolBeam      src/Polbeam/polbeam.hpp:23-93    (owns its maps)
  *   GPUConvolver src/GpuConvolver/gpuconvolver.hpp:17-82
  *   Convolver    src/Convolver/convolver.hpp:14-36 (computed on the GPU)
+ *   SkyMap       src/SkyMap/skymap.hpp:13-55      (normal equations on the GPU)
+ *   BPoint       src/Bpoint/BPoint.h:30-103, focal_plane_to_sky only
  *   CUDACONV::RunConfig src/GpuConvolver/cuda/cudaConv2.h:13-26
  * Errors are exceptions: std::invalid_argument where the reference threw it
  * (invalid configuration, gpuconvolver.cu:74-78; bad polFlag,
@@ -260,6 +262,76 @@ class Convolver {
     GPUConvolver gpu;
 };
 
+/* SkyMap (src/SkyMap/skymap.hpp:13-55): binned I, Q, U map-maker whose
+ * normal equations live on a GPU; the accumulators start at zero. */
+class SkyMap {
+  public:
+    explicit SkyMap(int nside_, int device = 0)
+        : nside(nside_), nPixels(12L * nside_ * nside_)
+    {
+        check(pisco_skymap_create(device, nside, &map));
+        stokesI.assign((size_t)nPixels, 0.0f);
+        stokesQ.assign((size_t)nPixels, 0.0f);
+        stokesU.assign((size_t)nPixels, 0.0f);
+        hitsMap.assign((size_t)nPixels, 0);
+    }
+    ~SkyMap() { pisco_skymap_destroy(map); }
+    SkyMap(const SkyMap &) = delete;
+    SkyMap &operator=(const SkyMap &) = delete;
+
+    /* skymap.cpp:68-132; arrays detector-major, ndetectors * nsamples */
+    void accumulate_data(int nsamples, float ra[], float dec[], float pa[], int ndetectors,
+                         float detPolAngles[], float data[])
+    {
+        check(pisco_skymap_accumulate(map, nsamples, ra, dec, pa, ndetectors, detPolAngles,
+                                      data));
+    }
+    /* skymap.cpp:134-231 */
+    void solve_map(void)
+    {
+        check(pisco_skymap_solve(map, stokesI.data(), stokesQ.data(), stokesU.data(),
+                                 hitsMap.data(), nullptr));
+    }
+    int get_nside(void) const { return nside; }
+    int size(void) const { return (int)nPixels; }
+    const float *get_stokes_I(void) { return stokesI.data(); }
+    const float *get_stokes_Q(void) { return stokesQ.data(); }
+    const float *get_stokes_U(void) { return stokesU.data(); }
+    const int *get_hitmap(void) { return hitsMap.data(); }
+
+  private:
+    pisco_skymap *map = nullptr;
+    int nside;
+    long nPixels;
+    std::vector<float> stokesI, stokesQ, stokesU;
+    std::vector<int> hitsMap;
+};
+
+/* BPoint::focal_plane_to_sky (src/Bpoint/bpoint.cpp:553-674) on the GPU,
+ * FP64; the rest of BPoint (SOFA astrometry) is not part of this library. */
+class BPoint {
+  public:
+    explicit BPoint(int deviceId = 0)
+    {
+        RunConfig c = Convolver::default_config();
+        c.deviceId = deviceId;
+        check(pisco_create(&c, &ctx));
+    }
+    ~BPoint() { pisco_destroy(ctx); }
+    BPoint(const BPoint &) = delete;
+    BPoint &operator=(const BPoint &) = delete;
+    void focal_plane_to_sky(int ndets, double det_dx[], double det_dy[], double det_pol_angle[],
+                            int nsamples, double x_ac[], double y_ac[], double rot_ac[],
+                            int flag, double x_det[], double y_det[], double rot_det[])
+    {
+        check(pisco_focal_plane_to_sky(ctx, ndets, det_dx, det_dy, det_pol_angle, nsamples,
+                                       x_ac, y_ac, rot_ac, flag, x_det, y_det, rot_det));
+    }
+
+  private:
+    pisco_ctx *ctx = nullptr;
+};
+
 } /* namespace pisco */
 
 #ifndef PISCO_NO_GLOBAL_NAMES
@@ -269,8 +341,10 @@ using RunConfig = pisco::RunConfig;
 using pisco::Convolver;
 using pisco::GPUConvolver;
 using pisco::PolBeam;
+using pisco::BPoint;
 using pisco::Scan;
 using pisco::Sky;
+using pisco::SkyMap;
 #endif
 
 #endif /* PISCO_HPP */

@@ -60,12 +60,14 @@
 #define O_SIN sin
 #define O_COS cos
 #define O_ACOS acos
+#define O_ASIN asin
 #define O_ATAN2 atan2
 #else
 #include "../pisco4cmb_amd/csrc/pisco_libm.h"
 #define O_SIN pl_sin
 #define O_COS pl_cos
 #define O_ACOS pl_acos
+#define O_ASIN pl_asin
 #define O_ATAN2 pl_atan2
 #endif
 
@@ -915,6 +917,7 @@ long or_disc_sizes(int nside_sky, double radius, long nsamples,
 void or_vec_sin(long n, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_SIN(x[i]); }
 void or_vec_cos(long n, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_COS(x[i]); }
 void or_vec_acos(long n, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_ACOS(x[i]); }
+void or_vec_asin(long n, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_ASIN(x[i]); }
 void or_vec_atan2(long n, const double *y, const double *x, double *o) { for (long i = 0; i < n; i++) o[i] = O_ATAN2(y[i], x[i]); }
 
 /* The kernels form phi = ((iphi - 0.5) * pi/2) / rdiv in the polar caps as
@@ -1105,4 +1108,50 @@ void or_dgesv3_many(long n, double *a, double *b, int *info)
 {
     for (long i = 0; i < n; i++)
         info[i] = o_dgesv3(a + 9 * i, b + 3 * i);
+}
+
+/* ------------------------------------------------------------------ */
+/* Focal plane to sky: BPoint::focal_plane_to_sky                      */
+/* ------------------------------------------------------------------ */
+
+/* src/Bpoint/bpoint.cpp:553-674: detector offsets (dx right, dy up, in the
+ * receiver frame looking out, radians) and polarisation angles, array-centre
+ * coordinates per sample -> detector coordinates per (detector, sample),
+ * outputs detector-major [ndets][nsamples].  flag 0: az/el (x to the West),
+ * 1: ra/dec (x to the East), 2: source-relative (x = -position angle).
+ * Every expression keeps the reference's operation order. */
+void or_focal_plane_to_sky(int ndets, const double *det_dx, const double *det_dy,
+                           const double *det_pol_angle, long nsamples, const double *x_ac,
+                           const double *y_ac, const double *rot_ac, int flag, double *x_det,
+                           double *y_det, double *rot_det)
+{
+    for (int det = 0; det < ndets; det++) {
+        double dx = det_dx[det];
+        double cdx = O_COS(dx), sdx = O_SIN(dx);
+        double cdy = O_COS(det_dy[det]), sdy = O_SIN(det_dy[det]);
+        double polang = det_pol_angle[det];
+        double cr = cdx * cdy;                         /* cos of the offset radius */
+        double sr = sqrt((1.0 - cr) * (1.0 + cr));
+        double alpha = O_ATAN2(sdy, sdx * cdy);        /* offset angle from +x */
+        for (long s = 0; s < nsamples; s++) {
+            long k = (long)det * nsamples + s;
+            double gamma = O_HALFPI - alpha - rot_ac[s];
+            double cg = O_COS(gamma);
+            double sy = O_SIN(y_ac[s]), cy = O_COS(y_ac[s]);
+            y_det[k] = O_ASIN(cr * sy + sr * cy * cg);
+            double seta = O_SIN(rot_ac[s]), ceta = O_COS(rot_ac[s]);
+            double cl = cy * ceta;
+            double sl_seps = cy * seta * cdx - sy * sdx;
+            double sl_ceps = sy * cdx + cy * seta * sdx;
+            double pa_det = O_ATAN2(sl_seps, cl * cdy - sl_ceps * sdy);
+            rot_det[k] = pa_det + polang;
+            if (flag == 2) {
+                x_det[k] = -pa_det;
+            } else {
+                double sg = O_SIN(gamma);
+                double dxd = O_ATAN2(sr * sg, cr * cy - sr * sy * cg);
+                x_det[k] = (flag == 0) ? x_ac[s] + dxd : x_ac[s] - dxd;
+            }
+        }
+    }
 }

@@ -73,6 +73,10 @@ SIGNATURES = {
                                           P, P, P, P]),
     "pisco_dev_batched_adjoint_map": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_long,
                                               P, P, P, P, P, P, P]),
+    "pisco_dev_batched_forward_pp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P, P, P,
+                                             c_long, P, P, P, P]),
+    "pisco_dev_batched_adjoint_map_pp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_long,
+                                                 P, P, P, P, P, P, P]),
     "pisco_skymap_create": (c_int, [c_int, c_int, ctypes.POINTER(c_void_p)]),
     "pisco_skymap_destroy": (None, [c_void_p]),
     "pisco_skymap_reset": (c_int, [c_void_p]),
@@ -84,6 +88,10 @@ SIGNATURES = {
     "pisco_skymap_solve": (c_int, [c_void_p, P, P, P, P, ctypes.POINTER(c_long)]),
     "pisco_dev_skymap_solve": (c_int, [c_void_p, c_void_p, P, P, P, P, ctypes.POINTER(c_long)]),
     "pisco_skymap_get_normal": (c_int, [c_void_p, P, P]),
+    "pisco_dev_focal_plane_to_sky": (c_int, [c_void_p, c_void_p, c_int, P, P, P, c_long, P, P, P,
+                                             c_int, P, P, P, P, P, P]),
+    "pisco_focal_plane_to_sky": (c_int, [c_void_p, c_int, P, P, P, c_long, P, P, P, c_int, P,
+                                         P, P]),
     "pisco_dev_pix2ang": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_ang2pix": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_get_interpol": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P, P]),

@@ -43,34 +43,46 @@ class DetectorBatch:
         check(lib.pisco_batch_create(self.ctx, self.npairs, table, ctypes.byref(h)))
         self.handle = h
 
-    def forward(self, I, Q, U, ra, dec, pa, nside_sky):
-        """TOD of every pair: float32 [npairs, 2, nsamples] on the device."""
-        I, Q, U = (_req(x, n) for x, n in ((I, "I"), (Q, "Q"), (U, "U")))
+    def _pointing(self, ra, dec, pa):
+        """(ra, dec, pa, nsamples, per_pair): [n] shared by every pair, or
+        [npairs, n] with a pointing per pair (caller's pair order)."""
         ra, dec, pa = (_req(x, n) for x, n in ((ra, "ra"), (dec, "dec"), (pa, "pa")))
+        if ra.shape != dec.shape or ra.shape != pa.shape:
+            raise ValueError("ra, dec and pa must have one shape")
+        if ra.dim() == 2:
+            if ra.shape[0] != self.npairs:
+                raise ValueError("a per-pair pointing must be [npairs, nsamples]")
+            return ra, dec, pa, ra.shape[1], True
+        return ra, dec, pa, ra.numel(), False
+
+    def forward(self, I, Q, U, ra, dec, pa, nside_sky):
+        """TOD of every pair: float32 [npairs, 2, nsamples] on the device.
+        ra, dec, pa: [nsamples] shared by the pairs, or [npairs, nsamples]
+        (e.g. from pointing.focal_plane_to_sky)."""
+        I, Q, U = (_req(x, n) for x, n in ((I, "I"), (Q, "Q"), (U, "U")))
+        ra, dec, pa, n, per_pair = self._pointing(ra, dec, pa)
         if I.numel() != 12 * nside_sky * nside_sky:
             raise ValueError("sky maps must have 12*nside_sky^2 pixels")
-        n = ra.numel()
         tod = torch.empty((self.npairs, 2, n), dtype=torch.float32, device=self.device)
-        check(lib.pisco_dev_batched_forward(self.ctx, _stream(self.device), self.handle,
-                                            nside_sky, I.data_ptr(), Q.data_ptr(),
-                                            U.data_ptr(), n, ra.data_ptr(), dec.data_ptr(),
-                                            pa.data_ptr(), tod.data_ptr()))
+        fn = lib.pisco_dev_batched_forward_pp if per_pair else lib.pisco_dev_batched_forward
+        check(fn(self.ctx, _stream(self.device), self.handle, nside_sky, I.data_ptr(),
+                 Q.data_ptr(), U.data_ptr(), n, ra.data_ptr(), dec.data_ptr(), pa.data_ptr(),
+                 tod.data_ptr()))
         return tod
 
     def adjoint_map(self, g, ra, dec, pa, nside_sky):
         """Sky gradient of <g, tod> summed over pairs: (gI, gQ, gU) float32
         maps; g is [npairs, 2, nsamples] like forward()'s output."""
         g = _req(g, "g")
-        ra, dec, pa = (_req(x, n) for x, n in ((ra, "ra"), (dec, "dec"), (pa, "pa")))
-        n = ra.numel()
+        ra, dec, pa, n, per_pair = self._pointing(ra, dec, pa)
         if g.numel() != self.npairs * 2 * n:
             raise ValueError("g must be [npairs, 2, nsamples]")
         npix = 12 * nside_sky * nside_sky
         out = [torch.empty(npix, dtype=torch.float32, device=self.device) for _ in range(3)]
-        check(lib.pisco_dev_batched_adjoint_map(self.ctx, _stream(self.device), self.handle,
-                                                nside_sky, n, ra.data_ptr(), dec.data_ptr(),
-                                                pa.data_ptr(), g.data_ptr(),
-                                                *[o.data_ptr() for o in out]))
+        fn = (lib.pisco_dev_batched_adjoint_map_pp if per_pair
+              else lib.pisco_dev_batched_adjoint_map)
+        check(fn(self.ctx, _stream(self.device), self.handle, nside_sky, n, ra.data_ptr(),
+                 dec.data_ptr(), pa.data_ptr(), g.data_ptr(), *[o.data_ptr() for o in out]))
         return tuple(out)
 
     def close(self):

@@ -1025,6 +1025,85 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
     return PISCO_OK;
 }
 
+int pisco_dev_batched_forward_pp(pisco_ctx *c, void *stream, const pisco_batch *b,
+                                 int nside_sky, const float *I, const float *Q, const float *U,
+                                 long nsamples, const float *ra, const float *dec,
+                                 const float *pa, float *tod)
+{
+    if (!c || !b || !I || !Q || !U || !tod || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (!valid_nside(nside_sky) || nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (b->device != c->device)
+        return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    HIPCHK(hipSetDevice(c->device));
+    BatchParams p;
+    CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
+    p.ptg_stride = nsamples;
+    p.tod = tod;
+    hipStream_t st = pick(c, stream);
+    long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : nsamples;
+    chunk = std::max<long>(chunk, std::min<long>(nsamples, 1L << 20));
+    for (long s0 = 0; s0 < nsamples; s0 += chunk) {
+        p.s_begin = s0;
+        p.s_end = std::min(nsamples, s0 + chunk);
+        HIPCHK(launch_batch_forward(1, p, st));   /* per-pair items: no shared geometry */
+    }
+    return PISCO_OK;
+}
+
+int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_batch *b,
+                                     int nside_sky, long nsamples, const float *ra,
+                                     const float *dec, const float *pa, const float *g,
+                                     float *gI, float *gQ, float *gU)
+{
+    if (!c || !b || !g || !gI || !gQ || !gU || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    if (!valid_nside(nside_sky) || nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "bad argument");
+    if (b->device != c->device)
+        return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = pick(c, stream);
+    const long npix = 12L * nside_sky * nside_sky;
+    CHK(c->acc.ensure(3 * (size_t)npix));
+    CHK(c->cot.ensure(std::max<long>(nsamples, 1)));
+    CHK(c->support.ensure(1));
+    HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
+    const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+    const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
+    /* one single-pair sky adjoint per pair, each with its own pointing and
+     * pa + pa_offset rounded to float (what the forward uses), all adding into
+     * one FP64 gradient */
+    for (int k = 0; k < b->npad && nsamples > 0; k++) {
+        const BatchPair &q = b->host[k];
+        if (q.out < 0)
+            continue;
+        const long po = (long)q.out * nsamples;
+        HIPCHK(launch_add_offset(pa + po, q.pa_off, c->cot.p, nsamples, st));
+        PairParams p;
+        CHK(setup_pair(c, nside_sky, nullptr, nullptr, nullptr, q.nside, q.npix, q.beam,
+                       nsamples, ra + po, dec + po, c->cot.p, p));
+        p.g_a = g + 2 * po;
+        p.g_b = g + 2 * po + nsamples;
+        p.gI = c->acc.p;
+        p.gQ = c->acc.p + npix;
+        p.gU = c->acc.p + 2 * npix;
+        if (c->adjmap_support) {
+            HIPCHK(launch_beam_support(q.beam, q.npix, c->support.p, st));
+            p.beam_last = c->support.p;
+        }
+        hipError_t e = launch_adjmap_lds(p, chunk, c->adjmap_cap, c->adjmap_rw, nb, st);
+        if (e != hipSuccess)
+            return fail(PISCO_ERR_DEVICE, "per-pair sky adjoint launch failed: %s",
+                        hipGetErrorString(e));
+    }
+    HIPCHK(launch_f64_to_f32(c->acc.p, gI, npix, st));
+    HIPCHK(launch_f64_to_f32(c->acc.p + npix, gQ, npix, st));
+    HIPCHK(launch_f64_to_f32(c->acc.p + 2 * npix, gU, npix, st));
+    return PISCO_OK;
+}
+
 int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch *b,
                                   int nside_sky, long nsamples, const float *ra,
                                   const float *dec, const float *pa, const float *g, float *gI,
@@ -1392,6 +1471,107 @@ int pisco_skymap_get_normal(pisco_skymap *m, double *AtA, double *AtD)
         AtD[3 * p + 2] = P[MM_SD][p];
     }
     return PISCO_OK;
+}
+
+} /* extern "C" */
+
+/* ======================================================================== */
+/* Detector pointing (BPoint::focal_plane_to_sky)                           */
+/* ======================================================================== */
+
+namespace {
+
+int fp_check(int ndets, long nsamples, int flag)
+{
+    if (ndets < 0 || nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "negative ndets or nsamples");
+    if (flag < 0 || flag > 2)
+        return fail(PISCO_ERR_INVALID, "flag must be 0 (az/el), 1 (ra/dec) or 2 (source)");
+    return PISCO_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pisco_dev_focal_plane_to_sky(pisco_ctx *c, void *stream, int ndets, const double *det_dx,
+                                 const double *det_dy, const double *det_pol_angle,
+                                 long nsamples, const double *x_ac, const double *y_ac,
+                                 const double *rot_ac, int flag, double *x_det, double *y_det,
+                                 double *rot_det, float *x32, float *y32, float *rot32)
+{
+    if (!c)
+        return fail(PISCO_ERR_INVALID, "NULL context");
+    CHK(fp_check(ndets, nsamples, flag));
+    if (ndets == 0 || nsamples == 0)
+        return PISCO_OK;
+    if (!det_dx || !det_dy || !det_pol_angle || !x_ac || !y_ac || !rot_ac)
+        return fail(PISCO_ERR_INVALID, "NULL input array");
+    HIPCHK(hipSetDevice(c->device));
+    FocalPlaneParams q;
+    q.ndets = ndets;
+    q.dx = det_dx;
+    q.dy = det_dy;
+    q.pol = det_pol_angle;
+    q.n = nsamples;
+    q.x_ac = x_ac;
+    q.y_ac = y_ac;
+    q.rot_ac = rot_ac;
+    q.flag = flag;
+    q.x = x_det;
+    q.y = y_det;
+    q.rot = rot_det;
+    q.x32 = x32;
+    q.y32 = y32;
+    q.rot32 = rot32;
+    hipError_t e = launch_focal_plane(q, pick(c, stream));
+    if (e != hipSuccess)
+        return fail(PISCO_ERR_DEVICE, "focal-plane launch failed: %s", hipGetErrorString(e));
+    return PISCO_OK;
+}
+
+int pisco_focal_plane_to_sky(pisco_ctx *c, int ndets, const double *det_dx,
+                             const double *det_dy, const double *det_pol_angle, long nsamples,
+                             const double *x_ac, const double *y_ac, const double *rot_ac,
+                             int flag, double *x_det, double *y_det, double *rot_det)
+{
+    if (!c)
+        return fail(PISCO_ERR_INVALID, "NULL context");
+    CHK(fp_check(ndets, nsamples, flag));
+    if (ndets == 0 || nsamples == 0)
+        return PISCO_OK;
+    if (!det_dx || !det_dy || !det_pol_angle || !x_ac || !y_ac || !rot_ac || !x_det ||
+        !y_det || !rot_det)
+        return fail(PISCO_ERR_INVALID, "NULL array");
+    HIPCHK(hipSetDevice(c->device));
+    const long nd = ndets, n = nsamples, no = nd * n;
+    DevBuf<double> buf;
+    CHK(buf.ensure(3 * nd + 3 * n + 3 * no));
+    double *d = buf.p;
+    hipStream_t st = c->stream;
+    int rc = PISCO_OK;
+    const double *src[6] = {det_dx, det_dy, det_pol_angle, x_ac, y_ac, rot_ac};
+    const long len[6] = {nd, nd, nd, n, n, n};
+    double *dst[6] = {d, d + nd, d + 2 * nd, d + 3 * nd, d + 3 * nd + n, d + 3 * nd + 2 * n};
+    double *out = d + 3 * nd + 3 * n;
+    for (int k = 0; k < 6 && rc == PISCO_OK; k++)
+        if (hipMemcpyAsync(dst[k], src[k], len[k] * sizeof(double), hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+            rc = fail(PISCO_ERR_DEVICE, "host-to-device copy failed");
+    if (rc == PISCO_OK)
+        rc = pisco_dev_focal_plane_to_sky(c, st, ndets, dst[0], dst[1], dst[2], nsamples, dst[3],
+                                          dst[4], dst[5], flag, out, out + no, out + 2 * no,
+                                          nullptr, nullptr, nullptr);
+    double *hout[3] = {x_det, y_det, rot_det};
+    for (int k = 0; k < 3 && rc == PISCO_OK; k++)
+        if (hipMemcpyAsync(hout[k], out + k * no, no * sizeof(double), hipMemcpyDeviceToHost,
+                           st) != hipSuccess)
+            rc = fail(PISCO_ERR_DEVICE, "device-to-host copy failed");
+    if (rc == PISCO_OK && hipStreamSynchronize(st) != hipSuccess)
+        rc = fail(PISCO_ERR_DEVICE, "stream synchronisation failed");
+    (void)hipStreamSynchronize(st);
+    buf.release();
+    return rc;
 }
 
 } /* extern "C" */

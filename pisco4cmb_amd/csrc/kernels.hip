@@ -468,9 +468,11 @@ __device__ inline PairParams batch_pair_params(const BatchParams &b, const Batch
     p.fc64.marg_k = 100.0 * 4.4e-16 * q.nside * 1.7320508075688772;
     p.beam = q.beam;
     p.beam_last = nullptr;
-    p.ra = b.ra;
-    p.dec = b.dec;
-    p.pa = b.pa;
+    /* a pointing per pair (ptg_stride > 0) is found by the pair's caller index */
+    const long po = (long)(q.out < 0 ? 0 : q.out) * b.ptg_stride;
+    p.ra = b.ra + po;
+    p.dec = b.dec + po;
+    p.pa = b.pa + po;
     p.s_begin = b.s_begin;
     p.s_end = b.s_end;
     const long row = 2 * (long)(q.out < 0 ? 0 : q.out) * b.nsamples;
@@ -505,7 +507,7 @@ void k_batch_pair(BatchParams b)
         return;   /* inert padding */
     const long i = b.s_begin + j;
     const PairParams p = batch_pair_params(b, q);
-    pair_sample<OP>(p, i, b.pa[i] + q.pa_off, tables[wave], lane);
+    pair_sample<OP>(p, i, p.pa[i] + q.pa_off, tables[wave], lane);
 }
 
 /* ---- batched forward: many detector pairs on one pointing ----------------
@@ -1118,7 +1120,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 if (q.out < 0)
                     continue;   /* inert padding (uniform per wave) */
                 pp = batch_pair_params(L.b, q);
-                pa_i = p.pa[i] + q.pa_off;
+                pa_i = pp.pa[i] + q.pa_off;
             } else {
                 pp = p;
                 pa_i = p.pa[i];
@@ -1216,6 +1218,13 @@ __global__ void k_fill_f64(double *x, long n)
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (long)gridDim.x * blockDim.x)
         x[i] = 0.0;
+}
+
+__global__ void k_add_offset(const float *x, float off, float *y, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = x[i] + off;
 }
 
 __global__ void k_f64_to_f32(const double *x, float *y, long n)
@@ -1506,6 +1515,14 @@ hipError_t launch_zero_f64(double *x, long n, hipStream_t st)
     if (n <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_fill_f64, dim3(gs_blocks(n)), dim3(256), 0, st, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_offset(const float *x, float off, float *y, long n, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_add_offset, dim3(gs_blocks(n)), dim3(256), 0, st, x, off, y, n);
     return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ struct BatchParams {
     FastConsts fc;            /* rho > 1e-6 band (omc_lo, omc_hi) */
     FastConsts64 fc64;        /* crho thresholds of the double-precision form */
     const float *ra, *dec, *pa;
+    long ptg_stride;          /* 0: shared pointing; else pair `out` at out * ptg_stride */
     long s_begin, s_end, nsamples;
     float *tod;               /* [out][2][nsamples] */
     const float *g;           /* cotangents, same layout */
@@ -116,6 +117,19 @@ hipError_t launch_map_solve(int npix, const double *acc, float *I, float *Q, flo
                             int *hits, unsigned long long *singular, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *x, long n, hipStream_t st);
 
+/* Detector pointing from array-centre pointing (focalplane.hip).  Outputs
+ * [ndets][n]; any output may be NULL. */
+struct FocalPlaneParams {
+    int ndets;
+    const double *dx, *dy, *pol;          /* [ndets] */
+    long n;
+    const double *x_ac, *y_ac, *rot_ac;   /* [n] */
+    int flag;                             /* 0 az/el, 1 ra/dec, 2 source-relative */
+    double *x, *y, *rot;
+    float *x32, *y32, *rot32;
+};
+hipError_t launch_focal_plane(const FocalPlaneParams &q, hipStream_t st);
+
 hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st);
 
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
@@ -134,6 +148,8 @@ hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStre
 hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
+/* y = x + off (float), n elements */
+hipError_t launch_add_offset(const float *x, float off, float *y, long n, hipStream_t st);
 hipError_t launch_pack_beam(long npix, const float *const in[8], float *out, hipStream_t st);
 
 hipError_t launch_pix2ang(const Hpx &h, long n, const int *pix, double *th, double *ph,

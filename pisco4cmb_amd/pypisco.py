@@ -27,7 +27,7 @@ import numpy as np
 from . import _lib
 from ._lib import RunConfig, check, lib, ptr_array
 
-__all__ = ["RunConfig", "Sky", "Scan", "PolBeam", "GPUConvolver", "Convolver", "SkyMap",
+__all__ = ["RunConfig", "Sky", "Scan", "PolBeam", "GPUConvolver", "Convolver", "SkyMap", "BPoint",
            "default_run_config", "beam_npix_for_radius", "ang2pix", "pix2ang"]
 
 
@@ -397,3 +397,37 @@ class SkyMap:
             self.close()
         except Exception:
             pass
+
+
+class BPoint:
+    """BPoint::focal_plane_to_sky (src/Bpoint/bpoint.cpp:553-674) computed on
+    the GPU in FP64 (the reference's SOFA-based astrometry is not part of
+    this library).  Outputs are filled in place, detector-major."""
+
+    def __init__(self, deviceId=0):
+        self._gpu = GPUConvolver(default_run_config(deviceId=deviceId))
+        self._gpu.configure_execution()
+
+    def focal_plane_to_sky(self, ndets, det_dx, det_dy, det_pol_angle, nsamples, x_ac, y_ac,
+                           rot_ac, flag, x_det, y_det, rot_det):
+        nd, n = int(ndets), int(nsamples)
+        ins = [np.ascontiguousarray(a, np.float64) for a in (det_dx, det_dy, det_pol_angle)]
+        ins += [np.ascontiguousarray(a, np.float64) for a in (x_ac, y_ac, rot_ac)]
+        for a, want in zip(ins, (nd, nd, nd, n, n, n)):
+            if a.size < want:
+                raise ValueError("input array too short")
+        outs = (x_det, y_det, rot_det)
+        for o in outs:
+            if (not isinstance(o, np.ndarray) or o.dtype != np.float64 or o.size < nd * n
+                    or not o.flags.c_contiguous):
+                raise ValueError("outputs must be C-contiguous float64 arrays of ndets*nsamples")
+        self._gpu._need_ctx()
+        rc = lib.pisco_focal_plane_to_sky(self._gpu.ctx, nd, *[_addr(a) for a in ins[:3]],
+                                          n, *[_addr(a) for a in ins[3:]], int(flag),
+                                          *[_addr(o) for o in outs])
+        if rc == _lib.PISCO_ERR_INVALID:
+            raise ValueError(lib.pisco_last_error().decode())
+        check(rc)
+
+    def close(self):
+        self._gpu.close()

@@ -56,10 +56,12 @@ class Oracle:
             "or_vec_sin": (None, [c_long, P, P]),
             "or_vec_cos": (None, [c_long, P, P]),
             "or_vec_acos": (None, [c_long, P, P]),
+            "or_vec_asin": (None, [c_long, P, P]),
             "or_vec_atan2": (None, [c_long, P, P, P]),
             "or_skymap_accumulate": (c_long, [c_int, c_long, P, P, P, c_long, c_int, P, P, P, P]),
             "or_skymap_solve": (c_long, [c_int, P, P, P, P, P, P]),
             "or_dgesv3_many": (None, [c_long, P, P, P]),
+            "or_focal_plane_to_sky": (None, [c_int, P, P, P, c_long, P, P, P, c_int, P, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -227,6 +229,16 @@ class Oracle:
         info = np.empty(a.shape[0], np.int32)
         self.L.or_dgesv3_many(a.shape[0], _a(a), _a(b), _a(info))
         return b, info
+
+    # --- BPoint
+    def focal_plane_to_sky(self, dx, dy, pol, x_ac, y_ac, rot_ac, flag):
+        dx, dy, pol = (np.ascontiguousarray(v, np.float64) for v in (dx, dy, pol))
+        x_ac, y_ac, rot_ac = (np.ascontiguousarray(v, np.float64) for v in (x_ac, y_ac, rot_ac))
+        nd, n = dx.size, x_ac.size
+        out = [np.empty((nd, n)) for _ in range(3)]
+        self.L.or_focal_plane_to_sky(nd, _a(dx), _a(dy), _a(pol), n, _a(x_ac), _a(y_ac),
+                                     _a(rot_ac), int(flag), *[_a(o) for o in out])
+        return out
 
     def vec(self, name, x, y=None):
         x = np.ascontiguousarray(x, np.float64)
