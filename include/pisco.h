@@ -115,7 +115,8 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
                             float *const ga[4], float *const gb[4]);
 
 /* Launch-shape tuning of a context (defaults are the measured best on the
- * NSIDE 512, 5-degree workload; results do not depend on them).  Keys: "adjmap_chunk" (samples per LDS window of the sky adjoint;
+ * NSIDE 512, 5-degree workload; except for beam_interpolation, results do
+ * not depend on them).  Keys: "adjmap_chunk" (samples per LDS window of the sky adjoint;
  * 0 selects direct global atomics), "adjmap_cap" (window pixels),
  * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
  * per CU), "adjmap_support" (1: windows cover only the discs of the beam's

@@ -683,7 +683,47 @@ typedef struct {
     long npix_beam;
     const float *a[3], *b[3];
     double rho_max;
+    int interp;   /* beam interpolated from the get_interpol neighbours */
 } conv_t;
+
+/* Beam interpolation mode of the forward and adjoints (or_set_beam_interpolation). */
+static int g_interp = 0;
+
+void or_set_beam_interpolation(int on) { g_interp = on != 0; }
+
+/* The beam at (rho, sigma) interpolated from its get_interpol neighbours
+ * inside the beam map, normalised by their weight sum: the interpolating
+ * form of convolver.cpp:124-147 (there commented out), in its precision:
+ * float values accumulated from double products, divided by the double
+ * weight sum; the values start from zero at every visit.  Returns 0 when no
+ * neighbour lies in the beam; nb, w, *ws describe the neighbours used. */
+static int o_interp_beam(const conv_t *cv, double rho, double sigma, float va[3], float vb[3],
+                         int nb[4], double w[4], double *ws_out)
+{
+    hpx_get_interpol(&cv->hb, rho, sigma, nb, w);
+    double ws = 0.0;
+    for (int b = 0; b < 3; b++) {
+        va[b] = 0.0f;
+        vb[b] = 0.0f;
+    }
+    for (int k = 0; k < 4; k++) {
+        if (nb[k] < cv->npix_beam) {
+            ws += w[k];
+            for (int b = 0; b < 3; b++) {
+                va[b] += (double)cv->a[b][nb[k]] * w[k];
+                vb[b] += (double)cv->b[b][nb[k]] * w[k];
+            }
+        }
+    }
+    *ws_out = ws;
+    if (!(ws > 0.0))
+        return 0;
+    for (int b = 0; b < 3; b++) {
+        va[b] /= ws;
+        vb[b] /= ws;
+    }
+    return 1;
+}
 
 /* Visit callback kinds */
 enum { V_FWD = 0, V_FWD64 = 1, V_ADJ_MAP = 2, V_ADJ_BEAM = 3 };
@@ -708,10 +748,23 @@ static void o_sample(const conv_t *cv, double ra_bc, double dec_bc,
                             dec_pix);
             double c2d = O_COS(2.0 * chi), s2d = O_SIN(2.0 * chi);
             float c2chi = (float)c2d, s2chi = (float)s2d;
-            int bpix = hpx_ang2pix(&cv->hb, rho, sigma);
-            if (bpix < cv->npix_beam) {
-                float ba0 = cv->a[0][bpix], ba1 = cv->a[1][bpix], ba2 = cv->a[2][bpix];
-                float bb0 = cv->b[0][bpix], bb1 = cv->b[1][bpix], bb2 = cv->b[2][bpix];
+            int bpix = 0, inside;
+            float va[3], vb[3];
+            int nb[4];
+            double w[4], ws = 0.0;
+            if (cv->interp) {
+                inside = o_interp_beam(cv, rho, sigma, va, vb, nb, w, &ws);
+            } else {
+                bpix = hpx_ang2pix(&cv->hb, rho, sigma);
+                inside = bpix < cv->npix_beam;
+                for (int b = 0; inside && b < 3; b++) {
+                    va[b] = cv->a[b][bpix];
+                    vb[b] = cv->b[b][bpix];
+                }
+            }
+            if (inside) {
+                float ba0 = va[0], ba1 = va[1], ba2 = va[2];
+                float bb0 = vb[0], bb1 = vb[1], bb2 = vb[2];
                 float sI = cv->I[skyPix], sQ = cv->Q[skyPix], sU = cv->U[skyPix];
                 if (kind == V_FWD) {
                     /* float expression, double accumulation (convolver.cpp:157-164) */
@@ -749,12 +802,20 @@ static void o_sample(const conv_t *cv, double ra_bc, double dec_bc,
                     double q = sQ, u = sU;
                     double v[6] = {ga * sI, ga * (q * c + u * s), ga * (-q * s + u * c),
                                    gb * sI, gb * (-q * c - u * s), gb * (q * s - u * c)};
-                    double *g = gbeam + 6 * (long)bpix;
-                    for (int k = 0; k < 6; k++) {
+                    /* nearest: one row; interpolated: each neighbour in the
+                     * beam with weight w / ws */
+                    for (int n = 0; n < (cv->interp ? 4 : 1); n++) {
+                        long row = cv->interp ? nb[n] : bpix;
+                        if (row >= cv->npix_beam)
+                            continue;
+                        double f = cv->interp ? w[n] / ws : 1.0;
+                        double *g = gbeam + 6 * row;
+                        for (int k = 0; k < 6; k++) {
 #ifdef _OPENMP
 #pragma omp atomic
 #endif
-                        g[k] += v[k];
+                            g[k] += v[k] * f;
+                        }
                     }
                 }
             }
@@ -778,6 +839,7 @@ static conv_t o_conv_make(int nside_sky, const float *I, const float *Q,
         cv.b[k] = beams[4 + k];
     }
     cv.rho_max = or_beam_rho_max(nside_beam, npix_beam);
+    cv.interp = g_interp;
     return cv;
 }
 

@@ -112,6 +112,7 @@ struct pisco_ctx {
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
     int batch_group = 4;          /* pairs per wave in batched forwards (1: per-pair items) */
     int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
+    int beam_interp = 0;          /* 1: beam interpolated from get_interpol neighbours */
     DevBuf<int> support;          /* last non-zero beam row (device) */
 };
 
@@ -200,8 +201,17 @@ int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
     for (long s = 0; s < n; s += chunk) {
         p.s_begin = s;
         p.s_end = std::min(n, s + chunk);
-        HIPCHK(launch_pair(op, p, st));
+        HIPCHK(c->beam_interp ? launch_pair_interp(op, p, st) : launch_pair(op, p, st));
     }
+    return PISCO_OK;
+}
+
+/* The batched launches compute the nearest-pixel convolution only. */
+int no_interp(pisco_ctx *c)
+{
+    if (c->beam_interp)
+        return fail(PISCO_ERR_INVALID,
+                    "beam interpolation is available for single-pair launches only");
     return PISCO_OK;
 }
 
@@ -286,6 +296,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 1 && value != 2 && value != 4)
             return fail(PISCO_ERR_INVALID, "batch_adj_group must be 1, 2 or 4");
         c->batch_adj_group = (int)value;
+    } else if (k == "beam_interpolation") {
+        if (value != 0 && value != 1)
+            return fail(PISCO_ERR_INVALID, "beam_interpolation must be 0 or 1");
+        c->beam_interp = (int)value;
     } else if (k == "adjmap_blocks") {
         if (value < 0 || value > 65536)
             return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
@@ -313,6 +327,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjmap_support;
     else if (k == "batch_group")
         *value = c->batch_group;
+    else if (k == "beam_interpolation")
+        *value = c->beam_interp;
     else if (k == "batch_adj_group")
         *value = c->batch_adj_group;
     else if (k == "cu_count")
@@ -717,7 +733,7 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
     p.gQ = c->acc.p + npix;
     p.gU = c->acc.p + 2 * npix;
     if (g_a || g_b) {
-        if (c->adjmap_chunk > 0) {
+        if (c->adjmap_chunk > 0 && !c->beam_interp) {
             int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
             if (c->adjmap_support) {
                 CHK(c->support.ensure(1));
@@ -1008,6 +1024,7 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
         return fail(PISCO_ERR_INVALID, "bad argument");
     if (b->device != c->device)
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    CHK(no_interp(c));
     HIPCHK(hipSetDevice(c->device));
     BatchParams p;
     CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
@@ -1036,6 +1053,7 @@ int pisco_dev_batched_forward_pp(pisco_ctx *c, void *stream, const pisco_batch *
         return fail(PISCO_ERR_INVALID, "bad argument");
     if (b->device != c->device)
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    CHK(no_interp(c));
     HIPCHK(hipSetDevice(c->device));
     BatchParams p;
     CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
@@ -1063,6 +1081,7 @@ int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_bat
         return fail(PISCO_ERR_INVALID, "bad argument");
     if (b->device != c->device)
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    CHK(no_interp(c));
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = pick(c, stream);
     const long npix = 12L * nside_sky * nside_sky;
@@ -1115,6 +1134,7 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
         return fail(PISCO_ERR_INVALID, "bad argument");
     if (b->device != c->device)
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
+    CHK(no_interp(c));
     HIPCHK(hipSetDevice(c->device));
     hipStream_t st = pick(c, stream);
     BatchParams bp;

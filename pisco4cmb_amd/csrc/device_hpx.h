@@ -423,16 +423,15 @@ __host__ __device__ inline Centre centre_make(float ra, float dec, float pa)
     return c;
 }
 
-/* Beam pixel (ang2pix of (rho, sigma) on the beam map) and chi for the sky
- * pixel at (ra_pix, cos/sin dec_pix): rho_sigma_chi_pix (sphtrigo.cpp:27-54)
- * followed by Healpix_Base::ang2pix (convolver.cpp:148-149). */
-__host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double ra_pix,
-                                 double cdp, double sdp, double &chi)
+/* rho, sigma and chi of the sky pixel at (ra_pix, cos/sin dec_pix) seen
+ * from the beam centre c: rho_sigma_chi_pix (sphtrigo.cpp:27-54). */
+__host__ __device__ inline void visit_rsc(const Centre &c, double ra_pix, double cdp, double sdp,
+                                          double &rho, double &sigma, double &chi)
 {
     double sd, cd;
     pl_sincos(ra_pix - c.ra, &sd, &cd);
     double crho = c.sdc * sdp + c.cdc * cdp * cd;
-    double rho = (crho >= 1.0) ? 0.0 : pl_acos(crho);
+    rho = (crho >= 1.0) ? 0.0 : pl_acos(crho);
     double alpha, beta;
     if (rho > 1.0e-6) {
         beta = pl_atan2(sd * c.cdc, c.sdc * cdp - c.cdc * sdp * cd);
@@ -441,7 +440,7 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
         beta = kHalfPi;
         alpha = kHalfPi;
     }
-    double sigma = kPi - alpha - c.psi;
+    sigma = kPi - alpha - c.psi;
     if (sigma < 0.0)
         sigma += 2.0 * kPi;
     else if (sigma > 2.0 * kPi)
@@ -449,6 +448,16 @@ __host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double
     chi = beta - sigma;
     if (chi < -kPi)
         chi += 2.0 * kPi;
+}
+
+/* Beam pixel (ang2pix of (rho, sigma) on the beam map) and chi for the sky
+ * pixel at (ra_pix, cos/sin dec_pix): rho_sigma_chi_pix followed by
+ * Healpix_Base::ang2pix (convolver.cpp:148-149). */
+__host__ __device__ inline int beam_pixel(const Hpx &hb, const Centre &c, double ra_pix,
+                                 double cdp, double sdp, double &chi)
+{
+    double rho, sigma;
+    visit_rsc(c, ra_pix, cdp, sdp, rho, sigma, chi);
     return ang2pix(hb, rho, sigma);
 }
 

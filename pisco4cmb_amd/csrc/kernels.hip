@@ -418,6 +418,120 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     pair_sample<OP>(p, i, p.pa[i], tables[wave], lane);
 }
 
+/* ---- beam interpolation mode ----------------------------------------------
+ *
+ * The convolution with the beam interpolated at (rho, sigma) from the four
+ * get_interpol neighbours that lie inside the beam map, normalised by the
+ * sum of their weights: the interpolating form of convolver.cpp:124-147
+ * (commented out there), in its precision (float values accumulated from
+ * double products, divided by the double weight sum), with the values
+ * starting from zero at every visit.  A visit with no neighbour in the beam
+ * contributes nothing.  The mapping is the exact FP64 rho_sigma_chi of the
+ * sky pixel; one wave per sample walks the disc as k_pair does, without the
+ * pipelining of the nearest-pixel kernels.  Sky adjoint and beam adjoint use
+ * FP64 atomics; the beam adjoint spreads each visit over the neighbours used
+ * with weights w / sum(w). */
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_pair_interp(PairParams p)
+{
+    __shared__ RunTable tables[kWavesPerBlock];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
+    const int lane = threadIdx.x % kWaveSize;
+    const long lb = xcd_remap(blockIdx.x, gridDim.x);
+    const long i = p.s_begin + lb * kWavesPerBlock + wave;
+    if (i >= p.s_end)
+        return;
+    RunTable &t = tables[wave];
+    Centre c;
+    SampleF sf;
+    SampleD sd;
+    Disc d;
+    sample_setup(p, i, p.pa[i], c, sf, sd, d);
+    double ga = 0.0, gb = 0.0;
+    if (OP != OP_FWD) {
+        ga = p.g_a ? (double)p.g_a[i] : 0.0;
+        gb = p.g_b ? (double)p.g_b[i] : 0.0;
+    }
+    double acc_a = 0.0, acc_b = 0.0;
+    for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+        const int total = build_runs(p.hs, d, r0, lane, t);
+        int s = 0;
+        for (int v = lane; v < total; v += kWaveSize) {
+            while (v >= t.pref[s + 1])
+                s++;
+            const int pix = t.pix0[s] + (v - t.pref[s]);
+            const int r = t.ring[s];
+            const RingRec rr = p.rings[r];
+            const double ra_pix = ring_phi(p.hs, rr, r, pix);
+            double rho, sigma, chi;
+            visit_rsc(c, ra_pix, rr.cdp, rr.sdp, rho, sigma, chi);
+            int nb[4];
+            double w[4];
+            get_interpol(p.hb, rho, sigma, nb, w);
+            float va[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            double ws = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if ((unsigned)nb[k] < (unsigned)p.npix_beam) {
+                    ws += w[k];
+                    const float *row = p.beam + 8 * (long)nb[k];
+#pragma unroll
+                    for (int b = 0; b < 6; b++)
+                        va[b] = (float)((double)va[b] + (double)row[b] * w[k]);
+                }
+            }
+            if (!(ws > 0.0))
+                continue;
+#pragma unroll
+            for (int b = 0; b < 6; b++)
+                va[b] = (float)((double)va[b] / ws);
+            double c2d, s2d;
+            pl_sincos(2.0 * chi, &s2d, &c2d);
+            const float c2 = (float)c2d, s2 = (float)s2d;
+            const float A0 = va[0], A1 = va[1], A2 = va[2];
+            const float B0 = va[3], B1 = va[4], B2 = va[5];
+            if (OP == OP_FWD) {
+                const float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
+                float ta = sI * (A0) + sQ * (A1 * c2 - A2 * s2) + sU * (A2 * c2 + A1 * s2);
+                float tb = sI * (B0) + sQ * (-B1 * c2 + B2 * s2) + sU * (-B2 * c2 - B1 * s2);
+                acc_a += ta;
+                acc_b += tb;
+            } else if (OP == OP_ADJ_MAP) {
+                double cc = c2, ss = s2;
+                unsafeAtomicAdd(&p.gI[pix], ga * A0 + gb * B0);
+                unsafeAtomicAdd(&p.gQ[pix], ga * (A1 * cc - A2 * ss) + gb * (-B1 * cc + B2 * ss));
+                unsafeAtomicAdd(&p.gU[pix], ga * (A2 * cc + A1 * ss) + gb * (-B2 * cc - B1 * ss));
+            } else {
+                double cc = c2, ss = s2;
+                double qq = p.Q[pix], u = p.U[pix], sI = p.I[pix];
+                const double v6[6] = {ga * sI, ga * (qq * cc + u * ss), ga * (-qq * ss + u * cc),
+                                      gb * sI, gb * (-qq * cc - u * ss), gb * (qq * ss - u * cc)};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if ((unsigned)nb[k] < (unsigned)p.npix_beam) {
+                        const double f = w[k] / ws;
+                        double *g = p.gbeam + 8 * (long)nb[k];
+#pragma unroll
+                        for (int b = 0; b < 6; b++)
+                            unsafeAtomicAdd(&g[b], v6[b] * f);
+                    }
+                }
+            }
+        }
+        wave_sync();
+    }
+    if (OP == OP_FWD) {
+        acc_a = wave_sum(acc_a);
+        acc_b = wave_sum(acc_b);
+        if (lane == 0) {
+            if (p.tod_a)
+                p.tod_a[i] = (float)acc_a;
+            if (p.tod_b)
+                p.tod_b[i] = (float)acc_b;
+        }
+    }
+}
+
 /* Item order of the batched launches: samples in chunks of kBatchChunk, and
  * within a chunk all items of one pair (or pair group) before the next.  The
  * XCD-aware block order hands each XCD whole chunks, so every XCD sees every
@@ -1401,6 +1515,25 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
     case OP_FWD: hipLaunchKernelGGL(k_pair<OP_FWD>, dim3(g), dim3(kBlock), 0, st, p); break;
     case OP_ADJ_MAP: hipLaunchKernelGGL(k_pair<OP_ADJ_MAP>, dim3(g), dim3(kBlock), 0, st, p); break;
     case OP_ADJ_BEAM: hipLaunchKernelGGL(k_pair<OP_ADJ_BEAM>, dim3(g), dim3(kBlock), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st)
+{
+    long n = p.s_end - p.s_begin;
+    if (n <= 0)
+        return hipSuccess;
+    unsigned g = nblocks(n, kWavesPerBlock);
+    switch (op) {
+    case OP_FWD: hipLaunchKernelGGL(k_pair_interp<OP_FWD>, dim3(g), dim3(kBlock), 0, st, p); break;
+    case OP_ADJ_MAP:
+        hipLaunchKernelGGL(k_pair_interp<OP_ADJ_MAP>, dim3(g), dim3(kBlock), 0, st, p);
+        break;
+    case OP_ADJ_BEAM:
+        hipLaunchKernelGGL(k_pair_interp<OP_ADJ_BEAM>, dim3(g), dim3(kBlock), 0, st, p);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

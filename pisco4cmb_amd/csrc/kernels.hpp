@@ -133,6 +133,8 @@ hipError_t launch_focal_plane(const FocalPlaneParams &q, hipStream_t st);
 hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st);
 
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
+/* The beam-interpolation mode of the same operations (k_pair_interp). */
+hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st);
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,

@@ -61,6 +61,7 @@ class Oracle:
             "or_skymap_accumulate": (c_long, [c_int, c_long, P, P, P, c_long, c_int, P, P, P, P]),
             "or_skymap_solve": (c_long, [c_int, P, P, P, P, P, P]),
             "or_dgesv3_many": (None, [c_long, P, P, P]),
+            "or_set_beam_interpolation": (None, [c_int]),
             "or_focal_plane_to_sky": (None, [c_int, P, P, P, c_long, P, P, P, c_int, P, P, P]),
         }
         for k, (r, a) in sig.items():
@@ -229,6 +230,11 @@ class Oracle:
         info = np.empty(a.shape[0], np.int32)
         self.L.or_dgesv3_many(a.shape[0], _a(a), _a(b), _a(info))
         return b, info
+
+    def set_beam_interpolation(self, on):
+        """Selects the oracle's beam interpolation mode for convolve and the
+        adjoints (process-wide; tests restore it)."""
+        self.L.or_set_beam_interpolation(int(bool(on)))
 
     # --- BPoint
     def focal_plane_to_sky(self, dx, dy, pol, x_ac, y_ac, rot_ac, flag):
