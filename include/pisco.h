@@ -296,6 +296,23 @@ int pisco_focal_plane_to_sky(pisco_ctx *ctx, int ndets, const double *det_dx,
                              const double *x_ac, const double *y_ac, const double *rot_ac,
                              int flag, double *x_det, double *y_det, double *rot_det);
 
+/* ---- text files of the reference's formats -------------------------------- */
+
+/* Reads a text file of ncols whitespace-separated numbers per row, the
+ * format numpy.savetxt writes for the reference's inputs (sky maps: rows of
+ * I Q U V per pixel, scripts/create_cmb.py:96-97; tilde beams: rows of
+ * I Q U V per beam pixel, scripts/fields2beams.py:175-176).  Column k of
+ * row r goes to cols[k][r] (a NULL column is skipped); values are rounded
+ * to float as strtof does.  Blank lines and lines starting with '#' are
+ * skipped.  *nrows receives the number of rows; with capacity 0 nothing is
+ * stored (a row count), otherwise a file of more than capacity rows is an
+ * error. */
+int pisco_read_txt_columns(const char *path, int ncols, long capacity, float *const cols[],
+                           long *nrows);
+/* Writes nrows rows of ncols columns as numpy.savetxt does ("%.18e",
+ * space-separated, one row per line). */
+int pisco_write_txt_columns(const char *path, int ncols, long nrows, const float *const cols[]);
+
 /* ---- HEALPix / matching primitives on the device (for verification) ---- */
 
 /* Healpix_Base::pix2ang for n pixels (device arrays). */

@@ -92,6 +92,9 @@ SIGNATURES = {
                                              c_int, P, P, P, P, P, P]),
     "pisco_focal_plane_to_sky": (c_int, [c_void_p, c_int, P, P, P, c_long, P, P, P, c_int, P,
                                          P, P]),
+    "pisco_read_txt_columns": (c_int, [ctypes.c_char_p, c_int, c_long, P,
+                                       ctypes.POINTER(c_long)]),
+    "pisco_write_txt_columns": (c_int, [ctypes.c_char_p, c_int, c_long, P]),
     "pisco_dev_pix2ang": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_ang2pix": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P]),
     "pisco_dev_get_interpol": (c_int, [c_void_p, c_void_p, c_int, c_long, P, P, P, P]),

@@ -84,6 +84,12 @@ template <typename T> struct DevBuf {
 
 } // namespace
 
+int pisco::set_error(int code, const char *msg)
+{
+    g_err = msg;
+    return code;
+}
+
 struct pisco_ctx {
     pisco_run_config cfg{};
     int device = 0;

@@ -172,6 +172,10 @@ FastConsts64 fast_consts64(const Hpx &hb);
 hipError_t launch_libm(int op, long n, const double *x, const double *y, double *o,
                        hipStream_t st);
 
+/* Records msg as the calling thread's pisco_last_error() and returns code
+ * (host code outside capi.hip). */
+int set_error(int code, const char *msg);
+
 } /* namespace pisco */
 
 #endif

@@ -21,6 +21,7 @@ RuntimeError; nothing exits the process.
 Arrays are numpy float32 (pointing in radians, pa counter-clockwise).
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -28,6 +29,7 @@ from . import _lib
 from ._lib import RunConfig, check, lib, ptr_array
 
 __all__ = ["RunConfig", "Sky", "Scan", "PolBeam", "GPUConvolver", "Convolver", "SkyMap", "BPoint",
+           "read_txt_columns", "write_txt_columns", "read_sky_txt", "write_sky_txt",
            "default_run_config", "beam_npix_for_radius", "ang2pix", "pix2ang"]
 
 
@@ -168,6 +170,27 @@ class PolBeam:
         check(lib.pisco_polbeam_build(self.nPixels, self.nside, int(nsideSky),
                                       self.epsilon, self.enabledDets.encode(),
                                       tilde, out))
+
+    def _tilde(self, polFlag):
+        if polFlag == "a":
+            return (self.Ia, self.Qa, self.Ua, self.Va)
+        if polFlag == "b":
+            return (self.Ib, self.Qb, self.Ub, self.Vb)
+        raise ValueError("polFlag must be 'a' or 'b'")
+
+    def read_tilde_beams(self, path, polFlag):
+        """tilde I, Q, U, V of detector polFlag from a text file of nPixels
+        rows, the file scripts/fields2beams.py writes (hpx_tilde_beams_*);
+        build() then normalises them."""
+        cols = read_txt_columns(path, 4)
+        if cols[0].size != self.nPixels:
+            raise ValueError("%s has %d rows, the beam has %d pixels"
+                             % (path, cols[0].size, self.nPixels))
+        for dst, src in zip(self._tilde(polFlag), cols):
+            dst[:] = src
+
+    def write_tilde_beams(self, path, polFlag):
+        write_txt_columns(path, self._tilde(polFlag))
 
     def make_unpol_gaussian_elliptical_beams(self, fwhmx, fwhmy, phi0):
         """Co-polarised elliptical Gaussian fields (polbeam.cpp:290-372); FWHM
@@ -431,3 +454,51 @@ class BPoint:
 
     def close(self):
         self._gpu.close()
+
+
+# ---- text files of the reference's formats -------------------------------
+
+def read_txt_columns(path, ncols):
+    """Columns of a whitespace-separated text file (numpy.savetxt format) as
+    float32 arrays, parsed natively (pisco_read_txt_columns)."""
+    n = ctypes.c_long()
+    p = str(path).encode()
+    rc = lib.pisco_read_txt_columns(p, int(ncols), 0, None, ctypes.byref(n))
+    if rc == _lib.PISCO_ERR_INVALID:
+        raise ValueError(lib.pisco_last_error().decode())
+    check(rc)
+    cols = [np.empty(n.value, np.float32) for _ in range(int(ncols))]
+    rc = lib.pisco_read_txt_columns(p, int(ncols), n.value, ptr_array([_addr(c) for c in cols]),
+                                    ctypes.byref(n))
+    if rc == _lib.PISCO_ERR_INVALID:
+        raise ValueError(lib.pisco_last_error().decode())
+    check(rc)
+    return cols
+
+
+def write_txt_columns(path, cols):
+    """Writes equal-length float32 columns as numpy.savetxt does."""
+    cols = [np.ascontiguousarray(c, np.float32) for c in cols]
+    if len({c.size for c in cols}) != 1:
+        raise ValueError("columns must have one length")
+    rc = lib.pisco_write_txt_columns(str(path).encode(), len(cols), cols[0].size,
+                                     ptr_array([_addr(c) for c in cols]))
+    if rc == _lib.PISCO_ERR_INVALID:
+        raise ValueError(lib.pisco_last_error().decode())
+    check(rc)
+
+
+def read_sky_txt(path):
+    """A Sky from a text map of I Q U V rows per RING pixel, the file
+    scripts/create_cmb.py writes (numpy.savetxt of [I, Q, U, V].T); NSIDE
+    from the row count."""
+    I, Q, U, V = read_txt_columns(path, 4)
+    nside = int(round(math.sqrt(I.size / 12.0)))
+    if 12 * nside * nside != I.size or nside < 1:
+        raise ValueError("%s has %d rows, not 12 nside^2" % (path, I.size))
+    return Sky(nside, I, Q, U, V)
+
+
+def write_sky_txt(path, sky):
+    write_txt_columns(path, [sky.sI, sky.sQ, sky.sU,
+                             sky.sV if sky.sV is not None else np.zeros_like(sky.sI)])
