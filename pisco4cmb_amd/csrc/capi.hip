@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pisco.h"
+#include "pisco.h"
 #include "device_hpx.h"
 #include "kernels.hpp"
 

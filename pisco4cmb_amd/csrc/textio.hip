@@ -19,7 +19,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/pisco.h"
+#include "pisco.h"
 #include "kernels.hpp"
 
 namespace {
