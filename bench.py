@@ -21,7 +21,7 @@ with |D_i| the exact disc sizes (device query_disc).  achieved = bytes of
 the dominant kernel / its mean duration (HIP events on the launch stream).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
-                  [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker]
+                  [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker] [--c5]
                   [--calib-copy-mib M]
 
 Also reported: the plain drop-in forward of the NSIDE 256 / 1.5 deg workload
@@ -73,6 +73,8 @@ def parse():
                    help="skip the plain drop-in forward of the NSIDE 256 workload")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched 64-detector workload")
+    p.add_argument("--c5", action="store_true",
+                   help="also time the large configuration (NSIDE 2048, 256 detectors)")
     p.add_argument("--no-mapmaker", action="store_true",
                    help="skip the binned map-maker workload")
     p.add_argument("--calib-copy-mib", type=int, default=0,
@@ -212,6 +214,8 @@ def main():
         out["plain_forward_nside256"] = plain_forward(ctx, stream, dev)
     if not args.no_batched:
         out["batched_64det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky)
+    if args.c5:
+        out["large_256det"] = large_workload(dev)
     if not args.no_mapmaker:
         out["map_maker_64det"] = map_maker_workload(dev, nside_sky, with_cpu=not args.no_cpu)
     if not args.no_cpu:
@@ -379,6 +383,76 @@ def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=
                               "hbm_frac": b_fa / (t_pf + t_pa) / HBM_PEAK},
             "batched_fwd_adj_vs_plain_forward": t_pf / t_step,
             "batched_fwd_adj_vs_plain_fwd_adj": (t_pf + t_pa) / t_step}
+
+
+def large_workload(dev, n=100_000, per_family=32, reps=1):
+    """The large configuration: sky NSIDE 2048 (power-law I, Q, U, l <= 128,
+    plus 200 point sources), 256 detectors as 128 pairs in four beam families
+    (FWHM 0.1, 0.2, 0.5, 1.0 deg; cap 3 x FWHM; beam NSIDE 8192, 8192, 4096,
+    2048; ellipticity U[0, 0.1]; per-pair position-angle offsets), 1e5 CES
+    samples shared by the pairs.  One batched forward and one batched sky
+    adjoint, timed with HIP events; gather-model bytes as the other legs."""
+    import numpy as np
+    import torch
+
+    from pisco4cmb_amd import inputs
+    from pisco4cmb_amd._lib import check, lib
+    from pisco4cmb_amd.batched import DetectorBatch
+    from pisco4cmb_amd import torch_ops
+    ns = 2048
+    I, Q, U, _ = inputs.powerlaw_sky(ns, seed=2048, lmax=128)
+    inputs.add_point_sources(I, ns, seed=2049)
+    ra, dec, pa = inputs.ces_scan(n, t0=1800.0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    It, Qt, Ut, rat, dect, pat = (t(x) for x in (I, Q, U, ra, dec, pa))
+    rng = np.random.default_rng(2050)
+    specs, radii = [], []
+    for fwhm, nb in ((0.1, 8192), (0.2, 8192), (0.5, 4096), (1.0, 2048)):
+        for _ in range(per_family):
+            b = inputs.gaussian_polbeam(nb, 3 * fwhm, fwhm, float(rng.uniform(0, 0.1)), 0.0,
+                                        nside_sky=ns)
+            specs.append((t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb,
+                          float(rng.uniform(-np.pi, np.pi))))
+            radii.append(b.get_rho_max())
+    batch = DetectorBatch(specs)
+    ctx = torch_ops._ctx(dev)
+    stream = torch_ops._stream(dev)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    visits = 0
+    for r in radii:
+        check(lib.pisco_dev_query_disc(ctx, stream, ns, n, rat.data_ptr(), dect.data_ptr(), r,
+                                       None, counts.data_ptr(), None))
+        visits += int(counts.to(torch.int64).sum().item())
+    g = torch.randn((len(specs), 2, n), device=dev)
+
+    def tm(f):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_f = tm(lambda: batch.forward(It, Qt, Ut, rat, dect, pat, ns))
+    t_a = tm(lambda: batch.adjoint_map(g, rat, dect, pat, ns))
+    batch.close()
+    det = 2 * len(specs) * n
+    b_f = visits * 36 + len(specs) * n * 20
+    b_a = visits * 48 + len(specs) * n * 20
+    return {"workload": "256 detectors (128 pairs, FWHM 0.1/0.2/0.5/1.0 deg x 32 pairs, cap "
+                        "3xFWHM, beam NSIDE 8192/8192/4096/2048), sky NSIDE 2048 power-law + 200 "
+                        "point sources, %d samples CES" % n,
+            "pair_visits": visits,
+            "batched_forward": {"det_samples_per_s": det / t_f, "s": t_f,
+                                "hbm_frac": b_f / t_f / HBM_PEAK},
+            "batched_adjoint": {"det_samples_per_s": det / t_a, "s": t_a,
+                                "hbm_frac": b_a / t_a / HBM_PEAK},
+            "fwd_adj": {"det_samples_per_s": det / (t_f + t_a), "s": t_f + t_a,
+                        "hbm_frac": (b_f + b_a) / (t_f + t_a) / HBM_PEAK}}
 
 
 def map_maker_workload(dev, nside_sky, n=250_000, ndet=64, reps=3, with_cpu=True,
