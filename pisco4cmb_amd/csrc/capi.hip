@@ -120,6 +120,7 @@ struct pisco_ctx {
     int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
     int beam_interp = 0;          /* 1: beam interpolated from get_interpol neighbours */
     DevBuf<int> support;          /* last non-zero beam row (device) */
+    DevBuf<float> sky4;           /* the forward's packed sky: I, Q, U, 0 per pixel */
 };
 
 namespace {
@@ -200,6 +201,14 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
 /* Launches op over all samples in chunks of ptgPerConv (or one launch). */
 int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
 {
+    if (op == OP_FWD && !c->beam_interp) {
+        /* the nearest-pixel forward reads the sky packed as one float4 per
+         * pixel: one load per visit instead of three */
+        const long npix = p.hs.npix;
+        CHK(c->sky4.ensure(4 * (size_t)npix));
+        HIPCHK(launch_pack_sky(p.I, p.Q, p.U, npix, (float4 *)c->sky4.p, st));
+        p.sky4 = (const float4 *)c->sky4.p;
+    }
     long n = p.s_end;
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : n;
     if (chunk < (1L << 20))
@@ -495,6 +504,7 @@ void pisco_destroy(pisco_ctx *ctx)
     ctx->acc.release();
     ctx->fout.release();
     ctx->support.release();
+    ctx->sky4.release();
     for (auto &kv : ctx->ring_tabs)
         (void)hipFree(kv.second);
     (void)hipStreamDestroy(ctx->stream);
@@ -1038,6 +1048,9 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
     p.ngroups = b->npad / G;
     p.tod = tod;
     hipStream_t st = pick(c, stream);
+    CHK(c->sky4.ensure(4 * (size_t)p.hs.npix));
+    HIPCHK(launch_pack_sky(I, Q, U, p.hs.npix, (float4 *)c->sky4.p, st));
+    p.sky4 = (const float4 *)c->sky4.p;
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : nsamples;
     chunk = std::max<long>(chunk, std::min<long>(nsamples, 1L << 20));
     for (long s0 = 0; s0 < nsamples; s0 += chunk) {
@@ -1066,6 +1079,9 @@ int pisco_dev_batched_forward_pp(pisco_ctx *c, void *stream, const pisco_batch *
     p.ptg_stride = nsamples;
     p.tod = tod;
     hipStream_t st = pick(c, stream);
+    CHK(c->sky4.ensure(4 * (size_t)p.hs.npix));
+    HIPCHK(launch_pack_sky(I, Q, U, p.hs.npix, (float4 *)c->sky4.p, st));
+    p.sky4 = (const float4 *)c->sky4.p;
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : nsamples;
     chunk = std::max<long>(chunk, std::min<long>(nsamples, 1L << 20));
     for (long s0 = 0; s0 < nsamples; s0 += chunk) {

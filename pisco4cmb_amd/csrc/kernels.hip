@@ -175,9 +175,10 @@ __device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, i
             q.v0 = *(const float4 *)row;
             q.v1 = *(const float2 *)(row + 4);
             if (LOAD_SKY) {
-                q.sI = p.I[pix];
-                q.sQ = p.Q[pix];
-                q.sU = p.U[pix];
+                const float4 sv = p.sky4[pix];
+                q.sI = sv.x;
+                q.sQ = sv.y;
+                q.sU = sv.z;
             }
             q.live = true;
             use(q);
@@ -292,9 +293,10 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
             q.v0 = *(const float4 *)row;
             q.v1 = *(const float2 *)(row + 4);
             const int sp = q.live ? pix : 0;
-            q.sI = p.I[sp];
-            q.sQ = p.Q[sp];
-            q.sU = p.U[sp];
+            const float4 sv = p.sky4[sp];
+            q.sI = sv.x;
+            q.sQ = sv.y;
+            q.sU = sv.z;
         } else if (q.live) {
             /* adjoints: only live visits load (measured faster there) */
             const float *row = p.beam + 8 * (long)bp;
@@ -566,6 +568,7 @@ __device__ inline PairParams batch_pair_params(const BatchParams &b, const Batch
     PairParams p;
     p.hs = b.hs;
     p.rings = b.rings;
+    p.sky4 = b.sky4;
     p.I = b.I;
     p.Q = b.Q;
     p.U = b.U;
@@ -849,10 +852,10 @@ __device__ inline void batch_walk(const BatchParams &p, BatchTable<G> &bt, const
             q.r = r;
             q.sp = startpix;
             if (LOAD_SKY) {
-                const int sp = act ? pix : 0;
-                q.sI = p.I[sp];
-                q.sQ = p.Q[sp];
-                q.sU = p.U[sp];
+                const float4 sv = p.sky4[act ? pix : 0];
+                q.sI = sv.x;
+                q.sQ = sv.y;
+                q.sU = sv.z;
             }
         }
         if (q.live)
@@ -915,7 +918,8 @@ void k_batch_fwd(BatchParams p)
                      const float2 &v1) {
         (void)r;
         (void)sp;
-        const float sI = p.I[pix], sQ = p.Q[pix], sU = p.U[pix];
+        const float4 sv = p.sky4[pix];
+        const float sI = sv.x, sQ = sv.y, sU = sv.z;
         const float ta =
             sI * v0.x + sQ * (v0.y * e.c2 - v0.z * e.s2) + sU * (v0.z * e.c2 + v0.y * e.s2);
         const float tb =
@@ -1341,6 +1345,13 @@ __global__ void k_add_offset(const float *x, float off, float *y, long n)
         y[i] = x[i] + off;
 }
 
+__global__ void k_pack_sky(const float *I, const float *Q, const float *U, long n, float4 *out)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = make_float4(I[i], Q[i], U[i], 0.f);
+}
+
 __global__ void k_f64_to_f32(const double *x, float *y, long n)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1656,6 +1667,15 @@ hipError_t launch_add_offset(const float *x, float off, float *y, long n, hipStr
     if (n <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_add_offset, dim3(gs_blocks(n)), dim3(256), 0, st, x, off, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_sky(const float *I, const float *Q, const float *U, long npix,
+                           float4 *sky4, hipStream_t st)
+{
+    if (npix <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_pack_sky, dim3(gs_blocks(npix)), dim3(256), 0, st, I, Q, U, npix, sky4);
     return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ struct PairParams {
     Hpx hs;
     const RingRec *rings;     /* per-ring constants, index 1..4N-1 */
     const float *I, *Q, *U;
+    const float4 *sky4;       /* forward: I, Q, U, 0 per pixel (packed from I, Q, U) */
     /* beam: first npix_beam RING pixels of a map of NSIDE hb.nside, rows of
      * eight floats A0 A1 A2 B0 B1 B2 A3 B3 (what the CPU forward reads comes
      * first: one 16-byte and one 8-byte load per visit) */
@@ -64,6 +65,7 @@ struct BatchParams {
     Hpx hs;
     const RingRec *rings;
     const float *I, *Q, *U;
+    const float4 *sky4;       /* forward: I, Q, U, 0 per pixel */
     const BatchPair *pairs;   /* device, sorted by rho_max descending */
     int npairs, ngroups;      /* ngroups = ceil(npairs / G) */
     FastConsts fc;            /* rho > 1e-6 band (omc_lo, omc_hi) */
@@ -149,6 +151,9 @@ int adjmap_describe(char *buf, size_t n, int cap, int rw);
 hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st);
 hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
+/* sky4[p] = (I[p], Q[p], U[p], 0) */
+hipError_t launch_pack_sky(const float *I, const float *Q, const float *U, long npix,
+                           float4 *sky4, hipStream_t st);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
 /* y = x + off (float), n elements */
 hipError_t launch_add_offset(const float *x, float off, float *y, long n, hipStream_t st);
