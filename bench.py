@@ -178,6 +178,18 @@ def main():
     ms_per_step = total_ms / args.steps
     value = k * n / (ms_per_step / 1e3)
 
+    # the deterministic sky adjoint (integer sums), outside the timed steps
+    check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 1))
+    adj()
+    d0 = torch.cuda.Event(enable_timing=True)
+    d1 = torch.cuda.Event(enable_timing=True)
+    d0.record()
+    adj()
+    d1.record()
+    torch.cuda.synchronize()
+    t_adj_det = d0.elapsed_time(d1) / 1e3
+    check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 0))
+
     dominant = "adjoint_map" if t_adj >= t_fwd else "forward"
     b_dom, t_dom = (b_adj, t_adj) if dominant == "adjoint_map" else (b_fwd, t_fwd)
     k_dom = "k_adjmap_lds" if dominant == "adjoint_map" else "k_pair<0>"
@@ -205,6 +217,7 @@ def main():
                      "traffic": measured_traffic(n, k_dom),
                      "kernel": k_dom, "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
         "fwd_adj": {"forward_s": t_fwd, "adjoint_s": t_adj,
+                    "adjoint_deterministic_s": t_adj_det,
                     "frac_hbm_fwd_plus_adj": (b_fwd + b_adj) / (t_fwd + t_adj) / HBM_PEAK,
                     "forward_tod_samples_per_s": k * n / t_fwd,
                     "wall_s_timed": wall, "setup_s": setup_s},

@@ -119,6 +119,9 @@ struct pisco_ctx {
     int batch_group = 4;          /* pairs per wave in batched forwards (1: per-pair items) */
     int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
     int beam_interp = 0;          /* 1: beam interpolated from get_interpol neighbours */
+    int adjmap_det = 0;           /* 1: deterministic sky adjoint (integer sums) */
+    DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
+    DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
     DevBuf<float> sky4;           /* the forward's packed sky: I, Q, U, 0 per pixel */
 };
@@ -221,6 +224,29 @@ int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
     return PISCO_OK;
 }
 
+/* Deterministic sky adjoint: the scale (device) for the given beams and
+ * cotangents, count = the number of contributions a pixel can receive at
+ * most (samples x pairs). */
+int det_scale(pisco_ctx *c, const float *const *beams, const long *npix, int nbeams,
+              const float *g_a, const float *g_b, long ng, long count, hipStream_t st,
+              const double **scale)
+{
+    CHK(c->fxwork.ensure(2));
+    CHK(c->fxscale.ensure(1));
+    HIPCHK(launch_fix_scale(beams, npix, nbeams, g_a, g_b, ng, count, c->fxwork.p, c->fxscale.p,
+                            st));
+    *scale = c->fxscale.p;
+    return PISCO_OK;
+}
+
+/* The window capacity that fits the LDS next to the per-wave tables. */
+int fit_cap(int cap, int rw, int group, bool det)
+{
+    while (cap > 256 && adjmap_lds_bytes(cap, rw, group, det) > 163840)
+        cap -= 256;
+    return cap;
+}
+
 /* The batched launches compute the nearest-pixel convolution only. */
 int no_interp(pisco_ctx *c)
 {
@@ -295,7 +321,7 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
     } else if (k == "adjmap_cap" || k == "adjmap_rw") {
         int cap = k == "adjmap_cap" ? (int)value : c->adjmap_cap;
         int rw = k == "adjmap_rw" ? (int)value : c->adjmap_rw;
-        if (value < 1 || adjmap_lds_bytes(cap, rw) > 163840)
+        if (value < 1 || adjmap_lds_bytes(cap, rw, 0, false) > 163840)
             return fail(PISCO_ERR_INVALID, "LDS window does not fit 160 KiB");
         c->adjmap_cap = cap;
         c->adjmap_rw = rw;
@@ -311,6 +337,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 1 && value != 2 && value != 4)
             return fail(PISCO_ERR_INVALID, "batch_adj_group must be 1, 2 or 4");
         c->batch_adj_group = (int)value;
+    } else if (k == "adjmap_deterministic") {
+        if (value != 0 && value != 1)
+            return fail(PISCO_ERR_INVALID, "adjmap_deterministic must be 0 or 1");
+        c->adjmap_det = (int)value;
     } else if (k == "beam_interpolation") {
         if (value != 0 && value != 1)
             return fail(PISCO_ERR_INVALID, "beam_interpolation must be 0 or 1");
@@ -344,6 +374,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->batch_group;
     else if (k == "beam_interpolation")
         *value = c->beam_interp;
+    else if (k == "adjmap_deterministic")
+        *value = c->adjmap_det;
     else if (k == "batch_adj_group")
         *value = c->batch_adj_group;
     else if (k == "cu_count")
@@ -505,6 +537,8 @@ void pisco_destroy(pisco_ctx *ctx)
     ctx->fout.release();
     ctx->support.release();
     ctx->sky4.release();
+    ctx->fxwork.release();
+    ctx->fxscale.release();
     for (auto &kv : ctx->ring_tabs)
         (void)hipFree(kv.second);
     (void)hipStreamDestroy(ctx->stream);
@@ -748,19 +782,32 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
     p.gI = c->acc.p;
     p.gQ = c->acc.p + npix;
     p.gU = c->acc.p + 2 * npix;
+    const bool det = c->adjmap_det != 0;
+    if (det && c->beam_interp)
+        return fail(PISCO_ERR_INVALID,
+                    "the deterministic sky adjoint is the nearest-pixel one: "
+                    "beam_interpolation must be 0");
+    const double *scale = nullptr;
+    if (det) {
+        const float *bl[1] = {beam8};
+        const long nl[1] = {npix_beam};
+        CHK(det_scale(c, bl, nl, 1, g_a, g_b, nsamples, nsamples, st, &scale));
+    }
     if (g_a || g_b) {
-        if (c->adjmap_chunk > 0 && !c->beam_interp) {
+        if ((c->adjmap_chunk > 0 || det) && !c->beam_interp) {
             int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+            const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
+            const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
             if (c->adjmap_support) {
                 CHK(c->support.ensure(1));
                 HIPCHK(launch_beam_support(beam8, npix_beam, c->support.p, st));
                 p.beam_last = c->support.p;
             }
-            hipError_t e =
-                launch_adjmap_lds(p, c->adjmap_chunk, c->adjmap_cap, c->adjmap_rw, nb, st);
+            hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, nullptr, 0.0, 1,
+                                             scale);
             if (e != hipSuccess) {
                 char d[256];
-                adjmap_describe(d, sizeof d, c->adjmap_cap, c->adjmap_rw);
+                adjmap_describe(d, sizeof d, cap, c->adjmap_rw);
                 return fail(PISCO_ERR_DEVICE, "sky adjoint launch failed: %s (%s)",
                             hipGetErrorString(e), d);
             }
@@ -768,9 +815,15 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
             CHK(run_pair(c, OP_ADJ_MAP, p, st));
         }
     }
-    HIPCHK(launch_f64_to_f32(p.gI, gI, npix, st));
-    HIPCHK(launch_f64_to_f32(p.gQ, gQ, npix, st));
-    HIPCHK(launch_f64_to_f32(p.gU, gU, npix, st));
+    if (det) {
+        HIPCHK(launch_fix_to_f32(p.gI, scale, gI, npix, st));
+        HIPCHK(launch_fix_to_f32(p.gQ, scale, gQ, npix, st));
+        HIPCHK(launch_fix_to_f32(p.gU, scale, gU, npix, st));
+    } else {
+        HIPCHK(launch_f64_to_f32(p.gI, gI, npix, st));
+        HIPCHK(launch_f64_to_f32(p.gQ, gQ, npix, st));
+        HIPCHK(launch_f64_to_f32(p.gU, gU, npix, st));
+    }
     return PISCO_OK;
 }
 
@@ -1113,6 +1166,19 @@ int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_bat
     HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
     const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
     const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
+    const bool det = c->adjmap_det != 0;
+    const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
+    const double *scale = nullptr;
+    if (det) {
+        std::vector<const float *> bl;
+        std::vector<long> nl;
+        for (int k = 0; k < b->npad; k++) {
+            bl.push_back(b->host[k].beam);
+            nl.push_back(b->host[k].out < 0 ? 0 : b->host[k].npix);
+        }
+        CHK(det_scale(c, bl.data(), nl.data(), (int)bl.size(), g, nullptr,
+                      2L * b->npairs * nsamples, (long)b->npairs * nsamples, st, &scale));
+    }
     /* one single-pair sky adjoint per pair, each with its own pointing and
      * pa + pa_offset rounded to float (what the forward uses), all adding into
      * one FP64 gradient */
@@ -1134,14 +1200,19 @@ int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_bat
             HIPCHK(launch_beam_support(q.beam, q.npix, c->support.p, st));
             p.beam_last = c->support.p;
         }
-        hipError_t e = launch_adjmap_lds(p, chunk, c->adjmap_cap, c->adjmap_rw, nb, st);
+        hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, nullptr, 0.0, 1,
+                                         scale);
         if (e != hipSuccess)
             return fail(PISCO_ERR_DEVICE, "per-pair sky adjoint launch failed: %s",
                         hipGetErrorString(e));
     }
-    HIPCHK(launch_f64_to_f32(c->acc.p, gI, npix, st));
-    HIPCHK(launch_f64_to_f32(c->acc.p + npix, gQ, npix, st));
-    HIPCHK(launch_f64_to_f32(c->acc.p + 2 * npix, gU, npix, st));
+    float *out[3] = {gI, gQ, gU};
+    for (int k = 0; k < 3; k++) {
+        if (det)
+            HIPCHK(launch_fix_to_f32(c->acc.p + k * npix, scale, out[k], npix, st));
+        else
+            HIPCHK(launch_f64_to_f32(c->acc.p + k * npix, out[k], npix, st));
+    }
     return PISCO_OK;
 }
 
@@ -1188,20 +1259,35 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     p.s_end = nsamples;
     const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
     const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
-    const int grp = c->batch_adj_group;
+    const bool det = c->adjmap_det != 0;
+    /* the deterministic mode takes one pair per wave (integer windows) */
+    const int grp = det ? 1 : c->batch_adj_group;
     bp.ngroups = b->npad / grp;
     /* the window capacity that fits next to the group's per-wave tables */
-    int cap = c->adjmap_cap;
-    while (cap > 256 && adjmap_lds_bytes(cap, c->adjmap_rw, grp) > 163840)
-        cap -= 256;
+    const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, grp, det);
+    const double *scale = nullptr;
+    if (det) {
+        std::vector<const float *> bl;
+        std::vector<long> nl;
+        for (int k = 0; k < b->npad; k++) {
+            bl.push_back(b->host[k].beam);
+            nl.push_back(b->host[k].out < 0 ? 0 : b->host[k].npix);
+        }
+        CHK(det_scale(c, bl.data(), nl.data(), (int)bl.size(), g, nullptr,
+                      2L * b->npairs * nsamples, (long)b->npairs * nsamples, st, &scale));
+    }
     hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, &bp,
-                                     std::min(b->rho_support, q0.rho_max), grp);
+                                     std::min(b->rho_support, q0.rho_max), grp, scale);
     if (e != hipSuccess)
         return fail(PISCO_ERR_DEVICE, "batched sky adjoint launch failed: %s",
                     hipGetErrorString(e));
-    HIPCHK(launch_f64_to_f32(bp.gI, gI, npix, st));
-    HIPCHK(launch_f64_to_f32(bp.gQ, gQ, npix, st));
-    HIPCHK(launch_f64_to_f32(bp.gU, gU, npix, st));
+    float *out[3] = {gI, gQ, gU};
+    for (int k = 0; k < 3; k++) {
+        if (det)
+            HIPCHK(launch_fix_to_f32(c->acc.p + k * npix, scale, out[k], npix, st));
+        else
+            HIPCHK(launch_f64_to_f32(c->acc.p + k * npix, out[k], npix, st));
+    }
     return PISCO_OK;
 }
 

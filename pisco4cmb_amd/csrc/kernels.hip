@@ -964,6 +964,10 @@ struct AdjLayout {
     int npairs;
     double rho_w, cos_w;
     BatchParams b;
+    /* deterministic mode: contributions are rounded to multiples of
+     * 1 / *fx_scale and summed as 64-bit integers (LDS windows and the global
+     * accumulators, whose FP64 planes then hold int64 bits) */
+    const double *fx_scale;
 };
 
 #ifndef PISCO_ADJ_WAVES
@@ -1000,8 +1004,12 @@ struct AdjWaveTable<1> {
 };
 
 /* BG: 0 one pair; 1 a batch, one wave per (pair, sample); 2, 4 a batch, one
- * wave per (group of BG pairs, sample) sharing each visit's geometry. */
-template <int BG>
+ * wave per (group of BG pairs, sample) sharing each visit's geometry.
+ * DET: deterministic mode (BG 0 and 1): every contribution is rounded once
+ * to a multiple of 1 / scale and all sums are 64-bit integer sums, which do
+ * not depend on the order of the additions; the result is the same bits for
+ * any schedule and launch shape. */
+template <int BG, bool DET = false>
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
 {
     constexpr bool BATCH = BG > 0;
@@ -1013,6 +1021,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     int *win_base = win_hi + L.rw;          /* rw + 1 */
     int *ctl = win_base + L.rw + 1;         /* 0 R0, 1 R1, 2 W, 3 fallback */
     float *acc = (float *)(ctl + 4);
+    /* deterministic mode: int64 windows, 8-byte aligned after ctl */
+    unsigned long long *acc64 =
+        (unsigned long long *)(((unsigned long)(ctl + 4) + 7ul) & ~7ul);
+    const double fxs = DET ? *L.fx_scale : 0.0;
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWaveSize);
@@ -1148,9 +1160,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             __syncthreads();
             const int W = ctl[2];
             fallback = W > L.cap;
-            if (!fallback)
-                for (int k = tid; k < 3 * W; k += kAdjBlock)
-                    acc[k] = 0.f;
+            if (!fallback) {
+                for (int k = tid; k < 3 * W; k += kAdjBlock) {
+                    if constexpr (DET)
+                        acc64[k] = 0ull;
+                    else
+                        acc[k] = 0.f;
+                }
+            }
             __syncthreads();
         }
 
@@ -1171,7 +1188,20 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 if (off < win_hi[k])   /* inside the ring's window */
                     slot = win_base[k] + off;
             }
-            if (slot >= 0) {
+            if constexpr (DET) {
+                const unsigned long long fI = (unsigned long long)(long long)rint((double)vI * fxs);
+                const unsigned long long fQ = (unsigned long long)(long long)rint((double)vQ * fxs);
+                const unsigned long long fU = (unsigned long long)(long long)rint((double)vU * fxs);
+                if (slot >= 0) {
+                    atomicAdd(&acc64[slot], fI);
+                    atomicAdd(&acc64[W + slot], fQ);
+                    atomicAdd(&acc64[2 * W + slot], fU);
+                } else {   /* no window holds the pixel: global int64 atomics */
+                    atomicAdd((unsigned long long *)&p.gI[pix], fI);
+                    atomicAdd((unsigned long long *)&p.gQ[pix], fQ);
+                    atomicAdd((unsigned long long *)&p.gU[pix], fU);
+                }
+            } else if (slot >= 0) {
                 atomicAdd(&acc[slot], vI);
                 atomicAdd(&acc[W + slot], vQ);
                 atomicAdd(&acc[2 * W + slot], vU);
@@ -1270,9 +1300,21 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
         /* flush the window */
         if (!fallback) {
             for (int slot = tid; slot < W; slot += kAdjBlock) {
-                float vI = acc[slot], vQ = acc[W + slot], vU = acc[2 * W + slot];
-                if (vI == 0.f && vQ == 0.f && vU == 0.f)
-                    continue;
+                float vI = 0.f, vQ = 0.f, vU = 0.f;
+                unsigned long long fI = 0, fQ = 0, fU = 0;
+                if constexpr (DET) {
+                    fI = acc64[slot];
+                    fQ = acc64[W + slot];
+                    fU = acc64[2 * W + slot];
+                    if ((fI | fQ | fU) == 0ull)
+                        continue;
+                } else {
+                    vI = acc[slot];
+                    vQ = acc[W + slot];
+                    vU = acc[2 * W + slot];
+                    if (vI == 0.f && vQ == 0.f && vU == 0.f)
+                        continue;
+                }
                 int lo = 0, hi = nring;   /* win_base[lo] <= slot < win_base[hi] */
                 while (hi - lo > 1) {
                     int mid = (lo + hi) >> 1;
@@ -1288,12 +1330,21 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 if (ip >= nr)
                     ip -= nr;
                 int pix = sp + ip;
-                if (vI != 0.f)
-                    unsafeAtomicAdd(&p.gI[pix], (double)vI);
-                if (vQ != 0.f)
-                    unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
-                if (vU != 0.f)
-                    unsafeAtomicAdd(&p.gU[pix], (double)vU);
+                if constexpr (DET) {
+                    if (fI)
+                        atomicAdd((unsigned long long *)&p.gI[pix], fI);
+                    if (fQ)
+                        atomicAdd((unsigned long long *)&p.gQ[pix], fQ);
+                    if (fU)
+                        atomicAdd((unsigned long long *)&p.gU[pix], fU);
+                } else {
+                    if (vI != 0.f)
+                        unsafeAtomicAdd(&p.gI[pix], (double)vI);
+                    if (vQ != 0.f)
+                        unsafeAtomicAdd(&p.gQ[pix], (double)vQ);
+                    if (vU != 0.f)
+                        unsafeAtomicAdd(&p.gU[pix], (double)vU);
+                }
             }
         }
         __syncthreads();
@@ -1350,6 +1401,68 @@ __global__ void k_pack_sky(const float *I, const float *Q, const float *U, long 
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (long)gridDim.x * blockDim.x)
         out[i] = make_float4(I[i], Q[i], U[i], 0.f);
+}
+
+/* Deterministic sky adjoint: out[0] = max over beam rows of the sum of |A0
+ * A1 A2 B0 B1 B2|, out[1] = max |g| (as float bits; atomicMax on the bits of
+ * non-negative floats orders them as the values). */
+__global__ void k_absmax_rows(const float *beam8, long npix, unsigned *out)
+{
+    float best = 0.f;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < npix; i += (long)gridDim.x * blockDim.x) {
+        const float *r = beam8 + 8 * i;
+        float v = fabsf(r[0]) + fabsf(r[1]) + fabsf(r[2]) + fabsf(r[3]) + fabsf(r[4]) +
+                  fabsf(r[5]);
+        best = fmaxf(best, v);
+    }
+    for (int m = kWaveSize / 2; m > 0; m >>= 1)
+        best = fmaxf(best, __shfl_xor(best, m, kWaveSize));
+    if (threadIdx.x % kWaveSize == 0)
+        atomicMax(out, __float_as_uint(best));
+}
+
+__global__ void k_absmax(const float *x, long n, unsigned *out)
+{
+    float best = 0.f;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        best = fmaxf(best, fabsf(x[i]));
+    for (int m = kWaveSize / 2; m > 0; m >>= 1)
+        best = fmaxf(best, __shfl_xor(best, m, kWaveSize));
+    if (threadIdx.x % kWaveSize == 0)
+        atomicMax(out, __float_as_uint(best));
+}
+
+/* scale = 2^k, the largest power of two with count * bound * scale <= 2^62,
+ * bound = max row sum * max |g| (the largest |contribution| of a visit) and
+ * count the most contributions one pixel can receive (one per sample and
+ * pair); every pixel's sum then fits a signed 64-bit integer. */
+__global__ void k_fix_scale(const unsigned *m, long count, double *scale)
+{
+    const double bound = (double)__uint_as_float(m[0]) * (double)__uint_as_float(m[1]) *
+                         (double)(count > 0 ? count : 1);
+    double s = 1.0;
+    if (bound > 0.0 && bound < INFINITY) {
+        int k = 62 - ilogb(bound) - 1;
+        k = k > 1000 ? 1000 : (k < -1000 ? -1000 : k);
+        s = ldexp(1.0, k);
+    }
+    *scale = s;
+}
+
+__global__ void k_fix_to_f32(const unsigned long long *x, const double *scale, float *y, long n)
+{
+    const double inv = 1.0 / *scale;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (float)((double)(long long)x[i] * inv);
+}
+
+__global__ void k_fill_u32(unsigned *x, int n, unsigned v)
+{
+    if ((int)threadIdx.x < n)
+        x[threadIdx.x] = v;
 }
 
 __global__ void k_f64_to_f32(const double *x, float *y, long n)
@@ -1569,33 +1682,35 @@ hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
-size_t adjmap_lds_bytes(int cap, int rw, int group)
+size_t adjmap_lds_bytes(int cap, int rw, int group, bool det)
 {
     const size_t wt = group == 2 ? sizeof(BatchTable<2>)
                       : group == 4 ? sizeof(BatchTable<4>)
                                    : sizeof(RunTable);
     return wt * kAdjWaves + sizeof(int) * (3 * (size_t)rw + 1 + 4) +
-           sizeof(float) * 3 * (size_t)cap;
+           (det ? 8 + 8 * 3 * (size_t)cap : sizeof(float) * 3 * (size_t)cap);
 }
 
-template <int BG>
+template <int BG, bool DET>
 static hipError_t launch_adjmap_bg(const PairParams &p, const AdjLayout &L, size_t bytes,
                                    long g, hipStream_t st)
 {
     static size_t configured = 0;
     if (bytes > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG, DET>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess)
             return e;
         configured = bytes;
     }
-    hipLaunchKernelGGL(k_adjmap_lds<BG>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, L);
+    hipLaunchKernelGGL((k_adjmap_lds<BG, DET>), dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p,
+                       L);
     return hipGetLastError();
 }
 
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
-                             hipStream_t st, const BatchParams *batch, double rho_w, int group)
+                             hipStream_t st, const BatchParams *batch, double rho_w, int group,
+                             const double *fx_scale)
 {
     long n = p.s_end - p.s_begin;
     if (n <= 0)
@@ -1614,13 +1729,20 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
         L.b = *batch;
     }
     const int bg = batch ? group : 0;
-    const size_t bytes = adjmap_lds_bytes(cap, rw, bg);
+    const bool det = fx_scale != nullptr;
+    if (det && bg > 1)
+        return hipErrorInvalidValue;   /* deterministic mode: one pair per wave */
+    L.fx_scale = fx_scale;
+    const size_t bytes = adjmap_lds_bytes(cap, rw, bg, det);
     const long g = std::min<long>(L.nchunks, nblocks);
+    if (det)
+        return bg == 0 ? launch_adjmap_bg<0, true>(p, L, bytes, g, st)
+                       : launch_adjmap_bg<1, true>(p, L, bytes, g, st);
     switch (bg) {
-    case 0: return launch_adjmap_bg<0>(p, L, bytes, g, st);
-    case 1: return launch_adjmap_bg<1>(p, L, bytes, g, st);
-    case 2: return launch_adjmap_bg<2>(p, L, bytes, g, st);
-    case 4: return launch_adjmap_bg<4>(p, L, bytes, g, st);
+    case 0: return launch_adjmap_bg<0, false>(p, L, bytes, g, st);
+    case 1: return launch_adjmap_bg<1, false>(p, L, bytes, g, st);
+    case 2: return launch_adjmap_bg<2, false>(p, L, bytes, g, st);
+    case 4: return launch_adjmap_bg<4, false>(p, L, bytes, g, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1628,14 +1750,15 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
 int adjmap_describe(char *buf, size_t n, int cap, int rw)
 {
     hipFuncAttributes a;
-    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds<0>);
+    hipError_t e = hipFuncGetAttributes(&a, (const void *)k_adjmap_lds<0, false>);
     if (e != hipSuccess)
         return snprintf(buf, n, "hipFuncGetAttributes: %s", hipGetErrorString(e));
     return snprintf(buf, n,
                     "block %d threads, kernel max %d, regs %d, static LDS %zu, dynamic LDS "
                     "%zu (max %d), private %zu",
                     kAdjBlock, a.maxThreadsPerBlock, a.numRegs, a.sharedSizeBytes,
-                    adjmap_lds_bytes(cap, rw, 0), a.maxDynamicSharedSizeBytes, a.localSizeBytes);
+                    adjmap_lds_bytes(cap, rw, 0, false), a.maxDynamicSharedSizeBytes,
+                    a.localSizeBytes);
 }
 
 hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st)
@@ -1676,6 +1799,36 @@ hipError_t launch_pack_sky(const float *I, const float *Q, const float *U, long 
     if (npix <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_pack_sky, dim3(gs_blocks(npix)), dim3(256), 0, st, I, Q, U, npix, sky4);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_scale(const float *const *beams, const long *npix, int nbeams,
+                            const float *g_a, const float *g_b, long nsamples, long count,
+                            unsigned *work, double *scale, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, work, 2, 0u);
+    for (int k = 0; k < nbeams; k++)
+        if (npix[k] > 0)
+            hipLaunchKernelGGL(k_absmax_rows, dim3(gs_blocks(npix[k])), dim3(256), 0, st,
+                               beams[k], npix[k], work);
+    if (g_a && nsamples > 0)
+        hipLaunchKernelGGL(k_absmax, dim3(gs_blocks(nsamples)), dim3(256), 0, st, g_a, nsamples,
+                           work + 1);
+    if (g_b && nsamples > 0)
+        hipLaunchKernelGGL(k_absmax, dim3(gs_blocks(nsamples)), dim3(256), 0, st, g_b, nsamples,
+                           work + 1);
+    hipLaunchKernelGGL(k_fix_scale, dim3(1), dim3(1), 0, st, (const unsigned *)work, count,
+                       scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_to_f32(const double *x, const double *scale, float *y, long n,
+                             hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fix_to_f32, dim3(gs_blocks(n)), dim3(256), 0, st,
+                       (const unsigned long long *)x, scale, y, n);
     return hipGetLastError();
 }
 

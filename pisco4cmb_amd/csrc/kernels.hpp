@@ -139,11 +139,25 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st);
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
+/* fx_scale (device, or NULL): deterministic mode with the scale of
+ * launch_fix_scale; the global accumulators then hold int64 bits */
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st, const BatchParams *batch = nullptr,
-                             double rho_w = 0.0, int group = 1);
-/* group: 0 one pair, 1 per-pair batch items, 2 or 4 pairs per wave */
-size_t adjmap_lds_bytes(int cap, int rw, int group = 0);
+                             double rho_w = 0.0, int group = 1,
+                             const double *fx_scale = nullptr);
+/* group: 0 one pair, 1 per-pair batch items, 2 or 4 pairs per wave; det: the
+ * deterministic mode's integer windows */
+size_t adjmap_lds_bytes(int cap, int rw, int group = 0, bool det = false);
+/* Deterministic sky adjoint: *scale (device) from the beams (beam8 rows)
+ * and the cotangents (g_a, g_b of nsamples floats, either may be NULL);
+ * count = the most contributions a pixel can receive; work holds 2 unsigned
+ * (device). */
+hipError_t launch_fix_scale(const float *const *beams, const long *npix, int nbeams,
+                            const float *g_a, const float *g_b, long nsamples, long count,
+                            unsigned *work, double *scale, hipStream_t st);
+/* y = int64 bits of x / scale */
+hipError_t launch_fix_to_f32(const double *x, const double *scale, float *y, long n,
+                             hipStream_t st);
 /* Launch resources of the sky-adjoint kernel, for error messages. */
 int adjmap_describe(char *buf, size_t n, int cap, int rw);
 /* *last = index of the last row of beam8 [npix][8] with a non-zero value in
