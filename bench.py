@@ -47,9 +47,10 @@ HBM_PEAK = 8.0e12   # bytes/s, MI355X
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_nside512_cap5_4e6.txt")
 
 
-def measured_traffic(samples, kernel):
-    """Calibrated HBM bytes per launch of `kernel` from the committed profile,
-    if it was measured on this sample count; else None."""
+def measured_traffic(samples, kernel, what="traffic_bytes_per_launch"):
+    """Calibrated HBM bytes per launch of `kernel` (or, with what =
+    "l2_hit_rate", its L2 hit rate) from the committed profile, if it was
+    measured on this sample count; else None."""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
@@ -57,7 +58,7 @@ def measured_traffic(samples, kernel):
     vals = dict(l.split(": ", 1) for l in lines if ": " in l and not l.startswith(" "))
     if vals.get("samples") != str(samples):
         return None
-    v = vals.get("traffic_bytes_per_launch %s" % kernel)
+    v = vals.get("%s %s" % (what, kernel))
     return float(v) if v else None
 
 
@@ -178,6 +179,8 @@ def main():
     ms_per_step = total_ms / args.steps
     value = k * n / (ms_per_step / 1e3)
 
+    tod_host = (tod_a[:args.cpu_samples].cpu().numpy(), tod_b[:args.cpu_samples].cpu().numpy())
+
     # the deterministic sky adjoint (integer sums), outside the timed steps
     check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 1))
     adj()
@@ -215,6 +218,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(n, k_dom),
+                     "l2_hit_rate": measured_traffic(n, k_dom, "l2_hit_rate"),
                      "kernel": k_dom, "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
         "fwd_adj": {"forward_s": t_fwd, "adjoint_s": t_adj,
                     "adjoint_deterministic_s": t_adj_det,
@@ -232,8 +236,11 @@ def main():
     if not args.no_mapmaker:
         out["map_maker_64det"] = map_maker_workload(dev, nside_sky, with_cpu=not args.no_cpu)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra,
-                                           dec, pa, g)
+        cb = cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g, tod_host)
+        parity = cb.pop("parity", None)
+        out["cpu_baseline"] = cb
+        if parity:
+            out["parity"] = parity
     print(json.dumps(out), flush=True)
 
 
@@ -537,9 +544,13 @@ def map_maker_workload(dev, nside_sky, n=250_000, ndet=64, reps=3, with_cpu=True
     return res
 
 
-def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g):
+def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g, tod_gpu=None):
     """The CPU oracle (a C restatement of the reference Convolver, OpenMP over
-    samples) on a prefix of the same workload: forward + adjoint-map."""
+    samples) on a prefix of the same workload: forward + adjoint-map.  With
+    the GPU TOD of the timed steps (tod_gpu = (a, b)), also the parity of
+    that TOD on the prefix: relative RMS against the CPU Convolver and
+    against the all-FP64 oracle, next to the CPU Convolver's own error
+    against the FP64 oracle."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind
     orc = oracle_bind.Oracle()
@@ -551,10 +562,29 @@ def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g):
     t1 = time.time()
     orc.adjoint_map(nside_sky, nside_beam, beams, ra[:m], dec[:m], pa[:m], g[0, :m], g[1, :m])
     t2 = time.time()
-    return {"value": 2 * m / (t2 - t0), "unit": "TOD samples/s", "cores": cores,
-            "kind": "port",
-            "sample": "first %d samples of the same workload, forward (%.1f s) + "
-                      "adjoint-map (%.1f s), OpenMP over samples" % (m, t1 - t0, t2 - t1)}
+    out = {"value": 2 * m / (t2 - t0), "unit": "TOD samples/s", "cores": cores,
+           "kind": "port",
+           "sample": "first %d samples of the same workload, forward (%.1f s) + "
+                     "adjoint-map (%.1f s), OpenMP over samples" % (m, t1 - t0, t2 - t1)}
+    if tod_gpu is not None:
+        import numpy as np
+        mp = min(m, 5000)
+        ca, cb = orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:mp], dec[:mp], pa[:mp],
+                              nthreads=cores)
+        da, db = orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:mp], dec[:mp], pa[:mp],
+                              precision64=True, nthreads=cores)
+
+        def rr(x, y):
+            x = np.concatenate([np.asarray(v, np.float64) for v in x])
+            y = np.concatenate([np.asarray(v, np.float64) for v in y])
+            return float(np.sqrt(np.mean((x - y) ** 2)) / np.sqrt(np.mean(y ** 2)))
+        ga, gb = (np.asarray(v[:mp]) for v in tod_gpu)
+        out["parity"] = {"samples": mp,
+                         "gpu_vs_cpu_convolver_rel_rms": rr((ga, gb), (ca, cb)),
+                         "gpu_vs_fp64_oracle_rel_rms": rr((ga, gb), (da, db)),
+                         "cpu_convolver_vs_fp64_oracle_rel_rms": rr((ca, cb), (da, db)),
+                         "tolerance": 1e-6}
+    return out
 
 
 if __name__ == "__main__":
