@@ -117,14 +117,19 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
 /* Launch-shape tuning of a context (defaults are the measured best on the
  * NSIDE 512, 5-degree workload; except for beam_interpolation, results do
  * not depend on them).  Keys: "adjmap_chunk" (samples per LDS window of the sky adjoint;
- * 0 selects direct global atomics), "adjmap_cap" (window pixels),
+ * 0 selects direct global atomics; -1, the default, picks 16 to 128 from the
+ * disc size for single-pair adjoints and 16 for batched ones), "adjmap_cap" (window pixels),
  * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
  * per CU), "adjmap_support" (1: windows cover only the discs of the beam's
  * non-zero rows, 0: the full rho_max discs; results are the same either way),
  * "batch_group" (batched forward: 1 = one wave per (pair, sample), the
  * single-pair computation; 2 or 4 = pairs per wave sharing each visit's
  * geometry), "batch_adj_group" (batched sky adjoint: 1, 2 or 4 pairs per
- * wave, as batch_group);
+ * wave, as batch_group), "forward_lanes" (single-pair forward: 64 = one
+ * sample per wave; 32, 16 or 8 lanes per sample with 64/lanes samples per
+ * wave; 0, the default, picks 8 below 1,000 expected pixels per disc, 16
+ * below 3,000, else 64), "adjmap_lanes" (single-pair sky adjoint: 64, 16 or
+ * 8; 0, the default, picks 8 below 1,000 pixels per disc, else 64);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);

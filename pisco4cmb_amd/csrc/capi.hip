@@ -111,7 +111,7 @@ struct pisco_ctx {
     std::map<int, RingRec *> ring_tabs;
     int ncu = 256;                /* compute units of the device */
     /* tuning (pisco_set_tuning) */
-    int adjmap_chunk = 16;        /* samples per LDS window; 0 = direct atomics */
+    int adjmap_chunk = -1;        /* samples per LDS window; 0 = direct atomics, -1 = auto */
     int adjmap_cap = 6144;        /* LDS window pixels */
     int adjmap_rw = 1024;         /* LDS window rings */
     int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
@@ -120,6 +120,8 @@ struct pisco_ctx {
     int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
     int beam_interp = 0;          /* 1: beam interpolated from get_interpol neighbours */
     int adjmap_det = 0;           /* 1: deterministic sky adjoint (integer sums) */
+    int forward_lanes = 0;        /* lanes per sample of the forward: 64, 32, 16, 8; 0 = auto */
+    int adjmap_lanes = 0;         /* lanes per sample of the sky adjoint: 64, 16, 8; 0 = auto */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
@@ -201,6 +203,36 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
     return PISCO_OK;
 }
 
+/* Expected sky pixels in a disc of radius rho_max: npix (1 - cos rho) / 2. */
+double disc_pixels(const PairParams &p)
+{
+    const double s = pl_sin(0.5 * std::min(p.rho_max, kPi));
+    return (double)p.hs.npix * s * s;
+}
+
+/* Lanes per sample of the nearest-pixel forward: the tuning value, or
+ * (auto) from the expected disc size.  Measured on MI355X over 1e6 CES
+ * samples: 8 lanes took 0.59x the time of 64 at 135 pixels per disc and
+ * 0.88x at 540; 16 lanes 0.98x at 1,500; 64 lanes were fastest at 6,000. */
+int fwd_lanes(const pisco_ctx *c, const PairParams &p)
+{
+    if (c->forward_lanes > 0)
+        return c->forward_lanes;
+    const double dp = disc_pixels(p);
+    return dp < 1000.0 ? 8 : dp < 3000.0 ? 16 : 64;
+}
+
+/* Lanes per sample of the single-pair sky adjoint: the tuning value, or
+ * (auto) from the expected disc size.  8 lanes took 0.80x the time of 64 at
+ * 135 pixels per disc and 0.95x at 540; at 1,500 pixels 16 lanes equalled
+ * 64 and 8 took 1.23x. */
+int adj_lanes(const pisco_ctx *c, const PairParams &p)
+{
+    if (c->adjmap_lanes > 0)
+        return c->adjmap_lanes;
+    return disc_pixels(p) < 1000.0 ? 8 : 64;
+}
+
 /* Launches op over all samples in chunks of ptgPerConv (or one launch). */
 int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
 {
@@ -212,6 +244,7 @@ int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
         HIPCHK(launch_pack_sky(p.I, p.Q, p.U, npix, (float4 *)c->sky4.p, st));
         p.sky4 = (const float4 *)c->sky4.p;
     }
+    const int lanes = (op == OP_FWD && !c->beam_interp) ? fwd_lanes(c, p) : 64;
     long n = p.s_end;
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : n;
     if (chunk < (1L << 20))
@@ -219,7 +252,12 @@ int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
     for (long s = 0; s < n; s += chunk) {
         p.s_begin = s;
         p.s_end = std::min(n, s + chunk);
-        HIPCHK(c->beam_interp ? launch_pair_interp(op, p, st) : launch_pair(op, p, st));
+        if (c->beam_interp)
+            HIPCHK(launch_pair_interp(op, p, st));
+        else if (lanes < 64)
+            HIPCHK(launch_fwd_sub(lanes, p, st));
+        else
+            HIPCHK(launch_pair(op, p, st));
     }
     return PISCO_OK;
 }
@@ -237,6 +275,24 @@ int det_scale(pisco_ctx *c, const float *const *beams, const long *npix, int nbe
                             st));
     *scale = c->fxscale.p;
     return PISCO_OK;
+}
+
+/* Samples per LDS window of a single-pair sky adjoint over discs of radius
+ * rho: the tuning value, or (auto) the largest power of two in [16, 128]
+ * that keeps a window at about 96,000 expected visits.  Each window costs
+ * three block-wide passes besides its visits; at 6,000 pixels per disc
+ * 16 samples per window measured fastest, at 135 pixels 128 samples took
+ * 0.71x the time of 16. */
+int adj_chunk(const pisco_ctx *c, int nside_sky, double rho)
+{
+    if (c->adjmap_chunk > 0)
+        return c->adjmap_chunk;
+    const double s = pl_sin(0.5 * std::min(rho, kPi));
+    const double disc = 12.0 * nside_sky * nside_sky * s * s;   /* npix (1 - cos rho) / 2 */
+    int chunk = 16;
+    while (chunk < 128 && 2.0 * chunk * disc <= 96000.0)
+        chunk *= 2;
+    return chunk;
 }
 
 /* The window capacity that fits the LDS next to the per-wave tables. */
@@ -315,8 +371,8 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         return fail(PISCO_ERR_INVALID, "NULL argument");
     std::string k(key);
     if (k == "adjmap_chunk") {
-        if (value < 0 || value > 4096)
-            return fail(PISCO_ERR_INVALID, "adjmap_chunk must be in [0, 4096]");
+        if (value < -1 || value > 4096)
+            return fail(PISCO_ERR_INVALID, "adjmap_chunk must be -1 (auto) or in [0, 4096]");
         c->adjmap_chunk = (int)value;
     } else if (k == "adjmap_cap" || k == "adjmap_rw") {
         int cap = k == "adjmap_cap" ? (int)value : c->adjmap_cap;
@@ -345,6 +401,14 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 0 && value != 1)
             return fail(PISCO_ERR_INVALID, "beam_interpolation must be 0 or 1");
         c->beam_interp = (int)value;
+    } else if (k == "forward_lanes") {
+        if (value != 0 && value != 8 && value != 16 && value != 32 && value != 64)
+            return fail(PISCO_ERR_INVALID, "forward_lanes must be 0 (auto), 8, 16, 32 or 64");
+        c->forward_lanes = (int)value;
+    } else if (k == "adjmap_lanes") {
+        if (value != 0 && value != 8 && value != 16 && value != 64)
+            return fail(PISCO_ERR_INVALID, "adjmap_lanes must be 0 (auto), 8, 16 or 64");
+        c->adjmap_lanes = (int)value;
     } else if (k == "adjmap_blocks") {
         if (value < 0 || value > 65536)
             return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
@@ -362,6 +426,10 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
     std::string k(key);
     if (k == "adjmap_chunk")
         *value = c->adjmap_chunk;
+    else if (k == "forward_lanes")
+        *value = c->forward_lanes;
+    else if (k == "adjmap_lanes")
+        *value = c->adjmap_lanes;
     else if (k == "adjmap_cap")
         *value = c->adjmap_cap;
     else if (k == "adjmap_rw")
@@ -794,9 +862,9 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
         CHK(det_scale(c, bl, nl, 1, g_a, g_b, nsamples, nsamples, st, &scale));
     }
     if (g_a || g_b) {
-        if ((c->adjmap_chunk > 0 || det) && !c->beam_interp) {
+        if ((c->adjmap_chunk != 0 || det) && !c->beam_interp) {
             int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
-            const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
+            const int chunk = adj_chunk(c, nside_sky, p.rho_max);
             const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
             if (c->adjmap_support) {
                 CHK(c->support.ensure(1));
@@ -804,7 +872,7 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
                 p.beam_last = c->support.p;
             }
             hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, nullptr, 0.0, 1,
-                                             scale);
+                                             scale, adj_lanes(c, p));
             if (e != hipSuccess) {
                 char d[256];
                 adjmap_describe(d, sizeof d, cap, c->adjmap_rw);
@@ -1165,7 +1233,6 @@ int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_bat
     CHK(c->support.ensure(1));
     HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
     const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
-    const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
     const bool det = c->adjmap_det != 0;
     const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
     const double *scale = nullptr;
@@ -1200,8 +1267,9 @@ int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_bat
             HIPCHK(launch_beam_support(q.beam, q.npix, c->support.p, st));
             p.beam_last = c->support.p;
         }
-        hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, nullptr, 0.0, 1,
-                                         scale);
+        hipError_t e = launch_adjmap_lds(p, adj_chunk(c, nside_sky, p.rho_max), cap,
+                                         c->adjmap_rw, nb, st, nullptr, 0.0, 1, scale,
+                                         adj_lanes(c, p));
         if (e != hipSuccess)
             return fail(PISCO_ERR_DEVICE, "per-pair sky adjoint launch failed: %s",
                         hipGetErrorString(e));

@@ -420,6 +420,200 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     pair_sample<OP>(p, i, p.pa[i], tables[wave], lane);
 }
 
+/* ---- small-disc forward: several samples per wave -------------------------
+ *
+ * For discs of a few hundred pixels the per-sample part of k_pair<OP_FWD>
+ * (beam-centre constants, disc set-up, the ring step, the final reduction)
+ * runs on all 64 lanes for one sample and costs more than the visits.  Here
+ * a wave holds G = 64/SUB samples of SUB lanes each: that part runs once for
+ * the G samples, each lane group walks its own disc SUB visits at a time, and
+ * a SUB-lane reduction gives each sample's TOD.  The mapping (certified FP64
+ * form), the per-visit values and the FP64 accumulation are those of
+ * k_pair<OP_FWD>. */
+template <int SUB>
+struct SubTable {
+    int pref[2 * SUB + 1];   /* visit offset of each run; pref[nrun] = total */
+    int pix0[2 * SUB];
+    int ring[2 * SUB];
+};
+
+template <int SUB>
+__device__ inline int seg_incl_scan(int x, int sl)
+{
+#pragma unroll
+    for (int d = 1; d < SUB; d <<= 1) {
+        int y = __shfl_up(x, d, SUB);
+        if (sl >= d)
+            x += y;
+    }
+    return x;
+}
+
+__device__ inline int wave_max_uniform(int x)
+{
+#pragma unroll
+    for (int m = kWaveSize / 2; m > 0; m >>= 1)
+        x = max(x, __shfl_xor(x, m, kWaveSize));
+    return __builtin_amdgcn_readfirstlane(x);
+}
+
+/* Builds the lane group's run table for its rings d.r_first + ps*SUB ..
+ * + SUB-1 (clipped to its nrings rings); returns the group's visit count. */
+template <int SUB>
+__device__ inline int build_sub(const Hpx &hs, const Disc &d, int nrings, int ps, int sl,
+                                SubTable<SUB> &t)
+{
+    const int r = d.r_first + ps * SUB + sl;
+    int p0[2] = {0, 0}, ln[2] = {0, 0};
+    int ns = 0;
+    if (ps * SUB + sl < nrings)
+        ns = disc_ring(hs, d, r, p0, ln);
+    const int tot = (ns > 0 ? ln[0] : 0) + (ns > 1 ? ln[1] : 0);
+    const int cnt_in = seg_incl_scan<SUB>(ns, sl);
+    const int tot_in = seg_incl_scan<SUB>(tot, sl);
+    const int cbase = cnt_in - ns;
+    int off = tot_in - tot;
+    for (int k = 0; k < ns; k++) {
+        t.pix0[cbase + k] = p0[k];
+        t.ring[cbase + k] = r;
+        t.pref[cbase + k] = off;
+        off += ln[k];
+    }
+    const int nrun = __shfl(cnt_in, SUB - 1, SUB);
+    const int total = __shfl(tot_in, SUB - 1, SUB);
+    if (sl == 0)
+        t.pref[nrun] = total;
+    wave_sync();
+    return total;
+}
+
+/* The walk of one ring pass of a lane group: its `total` visits, SUB at a
+ * time, for wtot (the wave's largest total) iterations' worth, with the
+ * software pipelining of walk_visits (next ring record prefetched; a visit's
+ * loads consumed one iteration later) and the certified FP64 mapping.  In the
+ * forward (LOAD_SKY) every load of the loop is issued unconditionally; the
+ * adjoints load the beam row of live visits only.  A lane without a visit
+ * computes the first visit of its run (or, in a group with no visit, pixel 0
+ * of ring 1) and discards it.  use(q) is called once per visit whose beam
+ * pixel lies inside the beam. */
+template <int SUB, bool LOAD_SKY, typename Use>
+__device__ inline void walk_sub(const PairParams &p, const SubTable<SUB> &t, int total,
+                                int wtot, int sl, const Centre &c, double c2psi, double s2psi,
+                                Use &use)
+{
+    if (wtot <= 0)
+        return;
+    const bool any = total > 0;
+    int s = 0;
+    int v = sl;
+    if (v < total)
+        while (v >= t.pref[s + 1])
+            s++;
+    RingRec rr = p.rings[any ? t.ring[s] : 1];
+    __builtin_amdgcn_s_waitcnt(0);   /* as in walk_visits */
+    Pending q;
+    q.live = false;
+    for (int v0 = 0; v0 < wtot; v0 += SUB, v += SUB) {
+        const bool act = v < total;
+        const int r = any ? t.ring[s] : 1;
+        const int pix = any ? t.pix0[s] + (act ? v - t.pref[s] : 0) : 0;
+        const RingRec cur = rr;
+        const int vn = v + SUB;
+        if (vn < total)
+            while (vn >= t.pref[s + 1])
+                s++;
+        rr = p.rings[any ? t.ring[s] : 1];
+        float c2, s2;
+        int bp = beam_pixel_f64(p.hb, p.fc64, c, c2psi, s2psi, ring_phi(p.hs, cur, r, pix),
+                                cur.cdp, cur.sdp, c2, s2);
+        bp = act ? bp : -1;
+        if (q.live)
+            use(q);
+        q.live = (unsigned)bp < (unsigned)p.npix_beam;
+        q.pix = pix;
+        q.r = r;
+        q.sp = cur.startpix;
+        q.bp = bp;
+        q.c2 = c2;
+        q.s2 = s2;
+        if constexpr (LOAD_SKY) {
+            const float *row = p.beam + 8 * (long)(q.live ? bp : 0);
+            q.v0 = *(const float4 *)row;
+            q.v1 = *(const float2 *)(row + 4);
+            const float4 sv = p.sky4[q.live ? pix : 0];
+            q.sI = sv.x;
+            q.sQ = sv.y;
+            q.sU = sv.z;
+        } else if (q.live) {
+            const float *row = p.beam + 8 * (long)bp;
+            q.v0 = *(const float4 *)row;
+            q.v1 = *(const float2 *)(row + 4);
+        }
+    }
+    if (q.live)
+        use(q);
+}
+
+/* Per-lane-group sample set-up of the several-samples-per-wave kernels:
+ * beam-centre constants, cos/sin 2 psi, the disc and its ring count (0 for a
+ * group without a sample). */
+__device__ inline int sub_setup(const PairParams &p, long i, bool valid, Centre &c,
+                                double &c2psi, double &s2psi, Disc &d)
+{
+    c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+    pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+    disc_setup(p.hs, kHalfPi - c.dec, c.ra, p.rho_max, p.cos_rmax, d);
+    return valid && d.r_last >= d.r_first ? d.r_last - d.r_first + 1 : 0;
+}
+
+template <int SUB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_fwd_sub(PairParams p)
+{
+    constexpr int G = kWaveSize / SUB;
+    __shared__ SubTable<SUB> tables[kWavesPerBlock * G];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
+    const int lane = threadIdx.x % kWaveSize;
+    const int grp = lane / SUB, sl = lane % SUB;
+    SubTable<SUB> &t = tables[wave * G + grp];
+    const long lb = xcd_remap(blockIdx.x, gridDim.x);
+    const long w0 = p.s_begin + (lb * kWavesPerBlock + wave) * G;   /* the wave's first sample */
+    if (w0 >= p.s_end)
+        return;   /* wave-uniform */
+    const long i = w0 + grp;
+    const bool valid = i < p.s_end;
+    Centre c;
+    double c2psi, s2psi;
+    Disc d;
+    const int nrings = sub_setup(p, valid ? i : w0, valid, c, c2psi, s2psi, d);
+    const int npass = (wave_max_uniform(nrings) + SUB - 1) / SUB;
+    double acc_a = 0.0, acc_b = 0.0;
+    auto use = [&](const Pending &e) {
+        const float A0 = e.v0.x, A1 = e.v0.y, A2 = e.v0.z;
+        const float B0 = e.v0.w, B1 = e.v1.x, B2 = e.v1.y;
+        const float c2 = e.c2, s2 = e.s2;
+        float ta = e.sI * (A0) + e.sQ * (A1 * c2 - A2 * s2) + e.sU * (A2 * c2 + A1 * s2);
+        float tb = e.sI * (B0) + e.sQ * (-B1 * c2 + B2 * s2) + e.sU * (-B2 * c2 - B1 * s2);
+        acc_a += ta;
+        acc_b += tb;
+    };
+    for (int ps = 0; ps < npass; ps++) {
+        const int total = build_sub<SUB>(p.hs, d, nrings, ps, sl, t);
+        walk_sub<SUB, true>(p, t, total, wave_max_uniform(total), sl, c, c2psi, s2psi, use);
+        wave_sync();
+    }
+#pragma unroll
+    for (int m = SUB / 2; m > 0; m >>= 1) {
+        acc_a += __shfl_xor(acc_a, m, SUB);
+        acc_b += __shfl_xor(acc_b, m, SUB);
+    }
+    if (valid && sl == 0) {
+        if (p.tod_a)
+            p.tod_a[i] = (float)acc_a;
+        if (p.tod_b)
+            p.tod_b[i] = (float)acc_b;
+    }
+}
+
 /* ---- beam interpolation mode ----------------------------------------------
  *
  * The convolution with the beam interpolated at (rho, sigma) from the four
@@ -1009,9 +1203,12 @@ struct AdjWaveTable<1> {
  * to a multiple of 1 / scale and all sums are 64-bit integer sums, which do
  * not depend on the order of the additions; the result is the same bits for
  * any schedule and launch shape. */
-template <int BG, bool DET = false>
+template <int BG, bool DET = false, int SUB = kWaveSize>
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
 {
+    static_assert(SUB == kWaveSize || (BG == 0 && sizeof(SubTable<SUB>) * (kWaveSize / SUB) <=
+                                                      sizeof(RunTable)),
+                  "several samples per wave: single pair, tables within the run table");
     constexpr bool BATCH = BG > 0;
     using WT = typename AdjWaveTable<BG>::type;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1096,12 +1293,15 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 win_hi[k] = -kBigI;
             }
             __syncthreads();
-            /* pass 1: windows */
-            for (int j = wave; j < ns; j += kAdjWaves) {
-                Centre c = centre_make(p.ra[s0 + j], p.dec[s0 + j], 0.f);
+            /* pass 1: windows (SUB lanes per sample, 64/SUB samples per wave) */
+            for (int j0 = wave * (kWaveSize / SUB); j0 < ns; j0 += kAdjWaves * (kWaveSize / SUB)) {
+                const int j = j0 + lane / SUB;
+                Centre c = centre_make(p.ra[s0 + (j < ns ? j : j0)], p.dec[s0 + (j < ns ? j : j0)],
+                                       0.f);
                 Disc d;
                 disc_setup(p.hs, kHalfPi - c.dec, c.ra, rho_w, cos_w, d);
-                for (int r = d.r_first + lane; r <= d.r_last; r += kWaveSize) {
+                const int r_end = j < ns ? d.r_last : d.r_first - 1;
+                for (int r = d.r_first + lane % SUB; r <= r_end; r += SUB) {
                     int lo, hi, ipix1, nr;
                     int kind = disc_ring_span(p.hs, d, r, lo, hi, ipix1, nr);
                     if (kind == 0)
@@ -1254,6 +1454,35 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                     scatter(pix, r, sp, vI, vQ, vU);
                 };
                 batch_walk<BG, false>(L.b, bt, d, lane, c, sf, K, d0, consume, exact);
+            }
+        } else if constexpr (SUB < kWaveSize) {
+            /* single pair, small discs: 64/SUB samples per wave, SUB lanes
+             * each, with the FP64 mapping (as k_fwd_sub) */
+            constexpr int G = kWaveSize / SUB;
+            const int grp = lane / SUB, sl = lane % SUB;
+            SubTable<SUB> &tg = ((SubTable<SUB> *)&wtab[wave])[grp];
+            for (int j0 = wave * G; j0 < ns; j0 += kAdjWaves * G) {
+                const int j = j0 + grp;
+                const bool valid = j < ns;
+                const long i = s0 + (valid ? j : j0);
+                Centre c;
+                double c2psi, s2psi;
+                Disc d;
+                const int nrings = sub_setup(p, i, valid, c, c2psi, s2psi, d);
+                const float ga = (valid && p.g_a) ? p.g_a[i] : 0.f;
+                const float gb = (valid && p.g_b) ? p.g_b[i] : 0.f;
+                auto use = [&](const Pending &q) {
+                    float vI, vQ, vU;
+                    adj_visit_values(ga, gb, q.v0, q.v1, q.c2, q.s2, vI, vQ, vU);
+                    scatter(q.pix, q.r, q.sp, vI, vQ, vU);
+                };
+                const int npass = (wave_max_uniform(nrings) + SUB - 1) / SUB;
+                for (int ps = 0; ps < npass; ps++) {
+                    const int total = build_sub<SUB>(p.hs, d, nrings, ps, sl, tg);
+                    walk_sub<SUB, false>(p, tg, total, wave_max_uniform(total), sl, c, c2psi,
+                                         s2psi, use);
+                    wave_sync();
+                }
             }
         } else {
         /* one wave per (pair, sample), pairs outermost */
@@ -1644,6 +1873,29 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st)
+{
+    long n = p.s_end - p.s_begin;
+    if (n <= 0)
+        return hipSuccess;
+    switch (sub) {
+    case 8:
+        hipLaunchKernelGGL(k_fwd_sub<8>, dim3(nblocks(n, kWavesPerBlock * 8)), dim3(kBlock), 0,
+                           st, p);
+        break;
+    case 16:
+        hipLaunchKernelGGL(k_fwd_sub<16>, dim3(nblocks(n, kWavesPerBlock * 4)), dim3(kBlock), 0,
+                           st, p);
+        break;
+    case 32:
+        hipLaunchKernelGGL(k_fwd_sub<32>, dim3(nblocks(n, kWavesPerBlock * 2)), dim3(kBlock), 0,
+                           st, p);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st)
 {
     long n = p.s_end - p.s_begin;
@@ -1691,26 +1943,26 @@ size_t adjmap_lds_bytes(int cap, int rw, int group, bool det)
            (det ? 8 + 8 * 3 * (size_t)cap : sizeof(float) * 3 * (size_t)cap);
 }
 
-template <int BG, bool DET>
+template <int BG, bool DET, int SUB = kWaveSize>
 static hipError_t launch_adjmap_bg(const PairParams &p, const AdjLayout &L, size_t bytes,
                                    long g, hipStream_t st)
 {
     static size_t configured = 0;
     if (bytes > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG, DET>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG, DET, SUB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess)
             return e;
         configured = bytes;
     }
-    hipLaunchKernelGGL((k_adjmap_lds<BG, DET>), dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p,
-                       L);
+    hipLaunchKernelGGL((k_adjmap_lds<BG, DET, SUB>), dim3((unsigned)g), dim3(kAdjBlock), bytes, st,
+                       p, L);
     return hipGetLastError();
 }
 
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st, const BatchParams *batch, double rho_w, int group,
-                             const double *fx_scale)
+                             const double *fx_scale, int lanes)
 {
     long n = p.s_end - p.s_begin;
     if (n <= 0)
@@ -1735,6 +1987,17 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
     L.fx_scale = fx_scale;
     const size_t bytes = adjmap_lds_bytes(cap, rw, bg, det);
     const long g = std::min<long>(L.nchunks, nblocks);
+    if (bg == 0 && lanes != kWaveSize) {   /* several samples per wave */
+        switch (lanes) {
+        case 8:
+            return det ? launch_adjmap_bg<0, true, 8>(p, L, bytes, g, st)
+                       : launch_adjmap_bg<0, false, 8>(p, L, bytes, g, st);
+        case 16:
+            return det ? launch_adjmap_bg<0, true, 16>(p, L, bytes, g, st)
+                       : launch_adjmap_bg<0, false, 16>(p, L, bytes, g, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (det)
         return bg == 0 ? launch_adjmap_bg<0, true>(p, L, bytes, g, st)
                        : launch_adjmap_bg<1, true>(p, L, bytes, g, st);

@@ -137,14 +137,19 @@ hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 /* The beam-interpolation mode of the same operations (k_pair_interp). */
 hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st);
+/* Forward with 64/sub samples per wave, sub = 16 or 32 lanes each (small
+ * discs); same results as launch_pair(OP_FWD). */
+hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st);
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
 /* fx_scale (device, or NULL): deterministic mode with the scale of
- * launch_fix_scale; the global accumulators then hold int64 bits */
+ * launch_fix_scale; the global accumulators then hold int64 bits.  lanes
+ * (single pair only): 64, or 8 or 16 lanes per sample with 64/lanes samples
+ * per wave (small discs) */
 hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, int nblocks,
                              hipStream_t st, const BatchParams *batch = nullptr,
                              double rho_w = 0.0, int group = 1,
-                             const double *fx_scale = nullptr);
+                             const double *fx_scale = nullptr, int lanes = 64);
 /* group: 0 one pair, 1 per-pair batch items, 2 or 4 pairs per wave; det: the
  * deterministic mode's integer windows */
 size_t adjmap_lds_bytes(int cap, int rw, int group = 0, bool det = false);

@@ -33,7 +33,7 @@ def rel_rms(a, b):
 def ctx():
     c = torch_ops._ctx(torch.device("cuda:0"))
     yield c
-    for k, v in ((b"adjmap_deterministic", 0), (b"adjmap_blocks", 0), (b"adjmap_chunk", 16)):
+    for k, v in ((b"adjmap_deterministic", 0), (b"adjmap_blocks", 0), (b"adjmap_chunk", -1)):
         check(lib.pisco_set_tuning(c, k, v))
 
 
