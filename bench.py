@@ -277,17 +277,37 @@ def plain_forward(ctx, stream, dev, n=1_000_000, reps=5):
                                     Ut.data_ptr(), nside_beam, npb, beam8.data_ptr(), n,
                                     rat.data_ptr(), dect.data_ptr(), pat.data_ptr(),
                                     ta.data_ptr(), tb.data_ptr()))
-    fwd()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        fwd()
-    e1.record()
-    torch.cuda.synchronize()
-    t_dev = e0.elapsed_time(e1) / reps / 1e3
+    gat, gbt = (t(x) for x in np.random.default_rng(78).normal(size=(2, n)).astype(np.float32))
+    gI, gQ, gU = (torch.empty(12 * nside_sky ** 2, dtype=torch.float32, device=dev)
+                  for _ in range(3))
+
+    def adj():
+        check(lib.pisco_dev_adjoint_map(ctx, stream, nside_sky, nside_beam, npb, beam8.data_ptr(),
+                                        n, rat.data_ptr(), dect.data_ptr(), pat.data_ptr(),
+                                        gat.data_ptr(), gbt.data_ptr(), gI.data_ptr(),
+                                        gQ.data_ptr(), gU.data_ptr()))
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+    t_dev = timed(fwd)
+    t_adj = timed(adj)
+    # the same calls with one sample per wave (the kernels the large discs use)
+    check(lib.pisco_set_tuning(ctx, b"forward_lanes", 64))
+    check(lib.pisco_set_tuning(ctx, b"adjmap_lanes", 64))
+    t_dev64, t_adj64 = timed(fwd), timed(adj)
+    check(lib.pisco_set_tuning(ctx, b"forward_lanes", 0))
+    check(lib.pisco_set_tuning(ctx, b"adjmap_lanes", 0))
     b = visits * 36 + n * 20
+    b_adj = visits * 48 + n * 20
 
     conv = pypisco.GPUConvolver(pypisco.default_run_config(ptgPerConv=n))
     conv.configure_execution()
@@ -309,7 +329,12 @@ def plain_forward(ctx, stream, dev, n=1_000_000, reps=5):
             "visits": visits, "mean_disc": visits / n,
             "device_resident": {"tod_samples_per_s": 2 * n / t_dev, "s": t_dev,
                                 "hbm_frac": b / t_dev / HBM_PEAK},
-            "host_api_with_pcie": {"tod_samples_per_s": 2 * n / t_host, "s": t_host}}
+            "host_api_with_pcie": {"tod_samples_per_s": 2 * n / t_host, "s": t_host},
+            "adjoint_device_resident": {"tod_samples_per_s": 2 * n / t_adj, "s": t_adj,
+                                        "hbm_frac": b_adj / t_adj / HBM_PEAK},
+            "fwd_adj_device_resident": {"tod_samples_per_s": 2 * n / (t_dev + t_adj),
+                                        "hbm_frac": (b + b_adj) / (t_dev + t_adj) / HBM_PEAK},
+            "one_sample_per_wave": {"forward_s": t_dev64, "adjoint_s": t_adj64}}
 
 
 def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=32, reps=3):
