@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 
@@ -1947,13 +1948,23 @@ template <int BG, bool DET, int SUB = kWaveSize>
 static hipError_t launch_adjmap_bg(const PairParams &p, const AdjLayout &L, size_t bytes,
                                    long g, hipStream_t st)
 {
-    static size_t configured = 0;
-    if (bytes > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG, DET, SUB>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    /* the dynamic-LDS limit already set for this kernel, per device */
+    constexpr int kDevs = 64;
+    static std::atomic<size_t> configured[kDevs];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= kDevs || bytes > configured[dev].load()) {
+        e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG, DET, SUB>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess)
             return e;
-        configured = bytes;
+        if (dev >= 0 && dev < kDevs) {
+            size_t cur = configured[dev].load();
+            while (bytes > cur && !configured[dev].compare_exchange_weak(cur, bytes)) {
+            }
+        }
     }
     hipLaunchKernelGGL((k_adjmap_lds<BG, DET, SUB>), dim3((unsigned)g), dim3(kAdjBlock), bytes, st,
                        p, L);

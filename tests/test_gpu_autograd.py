@@ -29,13 +29,18 @@ def rel_rms(a, b):
 
 
 @pytest.fixture(scope="module")
-def case():
+def case(oracle):
+    """Cotangents g = TOD + noise of the same size (TOD from the oracle):
+    <TOD, g> is then of the order of |TOD| |g|, so the dot-product checks
+    measure the identity, not the cancellation of an uncorrelated sum."""
     sky = inputs.iid_sky(NS, seed=21)
     beam = inputs.gaussian_polbeam(NB, 6.0, 2.0, 0.05, 0.01, nside_sky=NS)
     ra, dec, pa = inputs.raster_scan(600)
+    ta, tb = oracle.convolve(NS, *sky[:3], NB, list(beam.aBeams) + list(beam.bBeams), ra, dec,
+                             pa)
     rng = np.random.default_rng(5)
-    ga = rng.normal(size=ra.size).astype(np.float32)
-    gb = rng.normal(size=ra.size).astype(np.float32)
+    ga = (ta + ta.std() * rng.normal(size=ra.size)).astype(np.float32)
+    gb = (tb + tb.std() * rng.normal(size=ra.size)).astype(np.float32)
     return sky, beam, ra, dec, pa, ga, gb
 
 

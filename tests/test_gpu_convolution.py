@@ -106,9 +106,10 @@ def test_adjoint_map_matches_oracle_and_dot_product(c1, oracle):
     n = 3000
     ra, dec, pa = ra[:n], dec[:n], pa[:n]
     rng = np.random.default_rng(9)
-    ga = rng.normal(size=n).astype(np.float32)
-    gb = rng.normal(size=n).astype(np.float32)
     conv, a, b = run_gpu(128, sky, beam, ra, dec, pa)
+    # y = A x + noise: <A x, y> of the order of |A x| |y|, no cancellation
+    ga = (a + a.std() * rng.normal(size=n)).astype(np.float32)
+    gb = (b + b.std() * rng.normal(size=n)).astype(np.float32)
     gI, gQ, gU = conv.exec_adjoint_map(ga, gb)
     oI, oQ, oU = oracle.adjoint_map(128, 1024, beams_of(beam), ra, dec, pa, ga, gb)
     for g, o in ((gI, oI), (gQ, oQ), (gU, oU)):
