@@ -22,7 +22,10 @@ the dominant kernel / its mean duration (HIP events on the launch stream).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
                   [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker] [--c5]
-                  [--calib-copy-mib M]
+                  [--beam-adjoint-samples M] [--calib-copy-mib M]
+
+The beam adjoint (TOD -> beam maps) is timed on a prefix of the headline scan
+(default 2.5e5 samples), with B_adjb = sum_i |D_i| * 60 + N * 20.
 
 Also reported: the plain drop-in forward of the NSIDE 256 / 1.5 deg workload
 (device-resident and through the host API including PCIe copies), the
@@ -76,6 +79,8 @@ def parse():
                    help="skip the batched 64-detector workload")
     p.add_argument("--c5", action="store_true",
                    help="also time the large configuration (NSIDE 2048, 256 detectors)")
+    p.add_argument("--beam-adjoint-samples", type=int, default=250_000,
+                   help="prefix of the headline scan on which the beam adjoint is timed (0: skip)")
     p.add_argument("--no-mapmaker", action="store_true",
                    help="skip the binned map-maker workload")
     p.add_argument("--calib-copy-mib", type=int, default=0,
@@ -193,6 +198,30 @@ def main():
     t_adj_det = d0.elapsed_time(d1) / 1e3
     check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 0))
 
+    # the beam adjoint (TOD -> beam maps) on a prefix, outside the timed steps
+    nbp = min(n, args.beam_adjoint_samples)
+    beam_adj = None
+    if nbp > 0:
+        gbeam8 = torch.empty(npb * 8, dtype=torch.float32, device=dev)
+
+        def adjb():
+            check(lib.pisco_dev_adjoint_beam(ctx, stream, nside_sky, It.data_ptr(), Qt.data_ptr(),
+                                             Ut.data_ptr(), nside_beam, npb, nbp, rat.data_ptr(),
+                                             dect.data_ptr(), pat.data_ptr(), gat.data_ptr(),
+                                             gbt.data_ptr(), gbeam8.data_ptr()))
+        adjb()
+        d0.record()
+        adjb()
+        d1.record()
+        torch.cuda.synchronize()
+        t_adjb = d0.elapsed_time(d1) / 1e3
+        v_pre = int(counts[:nbp].to(torch.int64).sum().item())
+        b_adjb = v_pre * (12 + 24 * k) + nbp * (12 + 4 * k)
+        beam_adj = {"samples": nbp, "s": t_adjb, "tod_samples_per_s": k * nbp / t_adjb,
+                    "hbm_frac": b_adjb / t_adjb / HBM_PEAK,
+                    "note": "six FP64 global atomics per visit; bound by the atomic rate"}
+        del gbeam8
+
     dominant = "adjoint_map" if t_adj >= t_fwd else "forward"
     b_dom, t_dom = (b_adj, t_adj) if dominant == "adjoint_map" else (b_fwd, t_fwd)
     k_dom = "k_adjmap_lds" if dominant == "adjoint_map" else "k_pair<0>"
@@ -226,6 +255,8 @@ def main():
                     "forward_tod_samples_per_s": k * n / t_fwd,
                     "wall_s_timed": wall, "setup_s": setup_s},
     }
+    if beam_adj:
+        out["beam_adjoint"] = beam_adj
 
     if not args.no_forms:
         out["plain_forward_nside256"] = plain_forward(ctx, stream, dev)
