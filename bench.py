@@ -262,6 +262,8 @@ def main():
         out["plain_forward_nside256"] = plain_forward(ctx, stream, dev)
     if not args.no_batched:
         out["batched_64det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky)
+        out["batched_5deg_16det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky,
+                                                     npairs=8, five_deg=True)
     if args.c5:
         out["large_256det"] = large_workload(dev)
     if not args.no_mapmaker:
@@ -368,29 +370,37 @@ def plain_forward(ctx, stream, dev, n=1_000_000, reps=5):
             "one_sample_per_wave": {"forward_s": t_dev64, "adjoint_s": t_adj64}}
 
 
-def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=32, reps=3):
+def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=32, reps=3,
+                     five_deg=False):
     """64 detectors (32 pairs), each pair with its own elliptical Gaussian beam
     (FWHM U[0.2, 1.0] deg, ellipticity U[0, 0.1], cap 3 x FWHM, beam NSIDE
     2048) and position-angle offset, on the headline sky and a 2.5e5-sample
     CES scan: the batched forward + sky adjoint (one launch each, the step
     replayed from a hipGraph) against the plain form (one single-pair launch
-    per pair and operation)."""
+    per pair and operation).  five_deg: the pairs have the headline's beam
+    shape instead (cap 5 deg, beam NSIDE 4096, FWHM U[0.30, 0.36] deg,
+    ellipticity U[0, 0.1])."""
     import numpy as np
     import torch
 
     from pisco4cmb_amd import inputs, torch_ops
     from pisco4cmb_amd._lib import check, lib
     from pisco4cmb_amd.batched import BatchedStepGraph, DetectorBatch
-    rng = np.random.default_rng(64)
+    rng = np.random.default_rng(65 if five_deg else 64)
     ra, dec, pa = inputs.ces_scan(n, t0=3600.0)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     rat, dect, pat = t(ra), t(dec), t(pa)
     specs, radii = [], []
     for _ in range(npairs):
-        fwhm = float(rng.uniform(0.2, 1.0))
-        b = inputs.gaussian_polbeam(2048, 3 * fwhm, fwhm, float(rng.uniform(0, 0.1)), 0.0,
+        if five_deg:
+            fwhm, nb = float(rng.uniform(0.30, 0.36)), 4096
+            cap = 5.0
+        else:
+            fwhm, nb = float(rng.uniform(0.2, 1.0)), 2048
+            cap = 3 * fwhm
+        b = inputs.gaussian_polbeam(nb, cap, fwhm, float(rng.uniform(0, 0.1)), 0.0,
                                     nside_sky=nside_sky)
-        specs.append((t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), 2048,
+        specs.append((t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb,
                       float(rng.uniform(-np.pi, np.pi))))
         radii.append(b.get_rho_max())
     batch = DetectorBatch(specs)
@@ -429,13 +439,14 @@ def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=
     def plain_fwd():
         for k in range(npairs):
             check(lib.pisco_dev_forward(ctx, stream, nside_sky, It.data_ptr(), Qt.data_ptr(),
-                                        Ut.data_ptr(), 2048, b8[k].shape[0], b8[k].data_ptr(),
+                                        Ut.data_ptr(), specs[k][2], b8[k].shape[0],
+                                        b8[k].data_ptr(),
                                         n, rat.data_ptr(), dect.data_ptr(), pas[k].data_ptr(),
                                         ta.data_ptr(), tb.data_ptr()))
 
     def plain_adj():
         for k in range(npairs):
-            check(lib.pisco_dev_adjoint_map(ctx, stream, nside_sky, 2048, b8[k].shape[0],
+            check(lib.pisco_dev_adjoint_map(ctx, stream, nside_sky, specs[k][2], b8[k].shape[0],
                                             b8[k].data_ptr(), n, rat.data_ptr(),
                                             dect.data_ptr(), pas[k].data_ptr(),
                                             g[k, 0].data_ptr(), g[k, 1].data_ptr(),
@@ -446,9 +457,10 @@ def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=
     batch.close()
     det = 2 * npairs * n
     b_fa = visits * (36 + 48) + 2 * npairs * n * 20
-    return {"workload": "%d detectors (%d pairs), FWHM U[0.2,1.0] deg, cap 3xFWHM, beam NSIDE "
-                        "2048, sky NSIDE %d, %d samples CES, per-pair pa offsets"
-                        % (2 * npairs, npairs, nside_sky, n),
+    shape = ("FWHM U[0.30,0.36] deg, cap 5 deg, beam NSIDE 4096" if five_deg
+             else "FWHM U[0.2,1.0] deg, cap 3xFWHM, beam NSIDE 2048")
+    return {"workload": "%d detectors (%d pairs), %s, sky NSIDE %d, %d samples CES, per-pair "
+                        "pa offsets" % (2 * npairs, npairs, shape, nside_sky, n),
             "pair_visits": visits,
             "batched_fwd_adj_graph": {"det_samples_per_s": det / t_step, "s": t_step,
                                       "hbm_frac": b_fa / t_step / HBM_PEAK},

@@ -129,7 +129,10 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * sample per wave; 32, 16 or 8 lanes per sample with 64/lanes samples per
  * wave; 0, the default, picks 8 below 1,000 expected pixels per disc, 16
  * below 3,000, else 64), "adjmap_lanes" (single-pair sky adjoint: 64, 16 or
- * 8; 0, the default, picks 8 below 1,000 pixels per disc, else 64);
+ * 8; 0, the default, picks 8 below 1,000 pixels per disc, else 64),
+ * "batch_pair_major" (batched launches: 1 runs the pairs one after another
+ * with the single-pair kernels, 0 all pairs per chunk of samples; -1, the
+ * default, picks 1 when a pair's packed beam exceeds 4 MiB);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);

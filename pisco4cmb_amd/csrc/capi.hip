@@ -122,6 +122,7 @@ struct pisco_ctx {
     int adjmap_det = 0;           /* 1: deterministic sky adjoint (integer sums) */
     int forward_lanes = 0;        /* lanes per sample of the forward: 64, 32, 16, 8; 0 = auto */
     int adjmap_lanes = 0;         /* lanes per sample of the sky adjoint: 64, 16, 8; 0 = auto */
+    int batch_pair_major = -1;    /* batched launches pair by pair: 1 yes, 0 no, -1 auto */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
@@ -405,6 +406,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 0 && value != 8 && value != 16 && value != 32 && value != 64)
             return fail(PISCO_ERR_INVALID, "forward_lanes must be 0 (auto), 8, 16, 32 or 64");
         c->forward_lanes = (int)value;
+    } else if (k == "batch_pair_major") {
+        if (value < -1 || value > 1)
+            return fail(PISCO_ERR_INVALID, "batch_pair_major must be -1 (auto), 0 or 1");
+        c->batch_pair_major = (int)value;
     } else if (k == "adjmap_lanes") {
         if (value != 0 && value != 8 && value != 16 && value != 64)
             return fail(PISCO_ERR_INVALID, "adjmap_lanes must be 0 (auto), 8, 16 or 64");
@@ -430,6 +435,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->forward_lanes;
     else if (k == "adjmap_lanes")
         *value = c->adjmap_lanes;
+    else if (k == "batch_pair_major")
+        *value = c->batch_pair_major;
     else if (k == "adjmap_cap")
         *value = c->adjmap_cap;
     else if (k == "adjmap_rw")
@@ -1149,6 +1156,104 @@ int setup_batch(pisco_ctx *c, const pisco_batch *b, int nside_sky, const float *
     return PISCO_OK;
 }
 
+/* The deterministic mode's scale for a batch: every pair's beam, the
+ * cotangents of all pairs (2 npairs nsamples values), at most npairs
+ * nsamples contributions per pixel. */
+int batch_det_scale(pisco_ctx *c, hipStream_t st, const pisco_batch *b, long nsamples,
+                    const float *g, const double **scale)
+{
+    std::vector<const float *> bl;
+    std::vector<long> nl;
+    for (int k = 0; k < b->npad; k++) {
+        bl.push_back(b->host[k].beam);
+        nl.push_back(b->host[k].out < 0 ? 0 : b->host[k].npix);
+    }
+    return det_scale(c, bl.data(), nl.data(), (int)bl.size(), g, nullptr,
+                     2L * b->npairs * nsamples, (long)b->npairs * nsamples, st, scale);
+}
+
+/* Whether a batched launch runs pair by pair (the single-pair kernels, one
+ * pair after another) instead of all pairs per chunk of samples: the tuning
+ * value, or (auto) when a pair's packed beam (32 B per row) exceeds one XCD's
+ * 4 MiB L2.  Pairs interleaved per chunk gather from every beam at once; for
+ * 5-degree beams at NSIDE 4096 (12.2 MB each) that took 2.4x (forward) and
+ * 2.0x (sky adjoint) the time of running the pairs one after another. */
+bool pair_major(const pisco_ctx *c, const pisco_batch *b)
+{
+    if (c->batch_pair_major >= 0)
+        return c->batch_pair_major == 1;
+    for (const BatchPair &q : b->host)
+        if (q.out >= 0 && 32.0 * (double)q.npix > 4.0 * 1024 * 1024)
+            return true;
+    return false;
+}
+
+/* The forward of every pair of a batch, pair by pair, with the single-pair
+ * kernels: pair q reads its pointing at offset q.out * ptg_stride (0: one
+ * pointing shared by all pairs), with pa + q.pa_off rounded to float as the
+ * batched kernels form it, and writes TOD rows 2 q.out (a) and 2 q.out + 1
+ * (b). */
+int batch_forward_pairwise(pisco_ctx *c, hipStream_t st, const pisco_batch *b, int nside_sky,
+                           const float *I, const float *Q, const float *U, long nsamples,
+                           const float *ra, const float *dec, const float *pa, long ptg_stride,
+                           float *tod)
+{
+    CHK(c->cot.ensure(std::max<long>(nsamples, 1)));
+    for (const BatchPair &q : b->host) {
+        if (q.out < 0 || nsamples == 0)
+            continue;
+        const long po = (long)q.out * ptg_stride;
+        HIPCHK(launch_add_offset(pa + po, q.pa_off, c->cot.p, nsamples, st));
+        PairParams p;
+        CHK(setup_pair(c, nside_sky, I, Q, U, q.nside, q.npix, q.beam, nsamples, ra + po,
+                       dec + po, c->cot.p, p));
+        p.tod_a = tod + 2L * q.out * nsamples;
+        p.tod_b = p.tod_a + nsamples;
+        CHK(run_pair(c, OP_FWD, p, st));
+    }
+    return PISCO_OK;
+}
+
+/* The sky adjoint of every pair of a batch, pair by pair (single-pair LDS
+ * adjoints), all adding into the context's FP64 gradient (cleared by the
+ * caller); pointing as in batch_forward_pairwise; cotangent rows 2 q.out and
+ * 2 q.out + 1 of g; scale: the deterministic mode's scale or NULL. */
+int batch_adjoint_pairwise(pisco_ctx *c, hipStream_t st, const pisco_batch *b, int nside_sky,
+                           long nsamples, const float *ra, const float *dec, const float *pa,
+                           long ptg_stride, const float *g, const double *scale)
+{
+    const long npix = 12L * nside_sky * nside_sky;
+    const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+    const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, scale != nullptr);
+    CHK(c->cot.ensure(std::max<long>(nsamples, 1)));
+    CHK(c->support.ensure(1));
+    for (const BatchPair &q : b->host) {
+        if (q.out < 0 || nsamples == 0)
+            continue;
+        const long po = (long)q.out * ptg_stride;
+        HIPCHK(launch_add_offset(pa + po, q.pa_off, c->cot.p, nsamples, st));
+        PairParams p;
+        CHK(setup_pair(c, nside_sky, nullptr, nullptr, nullptr, q.nside, q.npix, q.beam,
+                       nsamples, ra + po, dec + po, c->cot.p, p));
+        p.g_a = g + 2L * q.out * nsamples;
+        p.g_b = p.g_a + nsamples;
+        p.gI = c->acc.p;
+        p.gQ = c->acc.p + npix;
+        p.gU = c->acc.p + 2 * npix;
+        if (c->adjmap_support) {
+            HIPCHK(launch_beam_support(q.beam, q.npix, c->support.p, st));
+            p.beam_last = c->support.p;
+        }
+        hipError_t e = launch_adjmap_lds(p, adj_chunk(c, nside_sky, p.rho_max), cap,
+                                         c->adjmap_rw, nb, st, nullptr, 0.0, 1, scale,
+                                         adj_lanes(c, p));
+        if (e != hipSuccess)
+            return fail(PISCO_ERR_DEVICE, "per-pair sky adjoint launch failed: %s",
+                        hipGetErrorString(e));
+    }
+    return PISCO_OK;
+}
+
 } // namespace
 
 int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, int nside_sky,
@@ -1163,6 +1268,9 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
     CHK(no_interp(c));
     HIPCHK(hipSetDevice(c->device));
+    if (pair_major(c, b))
+        return batch_forward_pairwise(c, pick(c, stream), b, nside_sky, I, Q, U, nsamples, ra,
+                                      dec, pa, 0, tod);
     BatchParams p;
     CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
     const int G = c->batch_group;
@@ -1195,6 +1303,9 @@ int pisco_dev_batched_forward_pp(pisco_ctx *c, void *stream, const pisco_batch *
         return fail(PISCO_ERR_INVALID, "batch and context are on different devices");
     CHK(no_interp(c));
     HIPCHK(hipSetDevice(c->device));
+    if (pair_major(c, b))
+        return batch_forward_pairwise(c, pick(c, stream), b, nside_sky, I, Q, U, nsamples, ra,
+                                      dec, pa, nsamples, tod);
     BatchParams p;
     CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
     p.ptg_stride = nsamples;
@@ -1232,48 +1343,14 @@ int pisco_dev_batched_adjoint_map_pp(pisco_ctx *c, void *stream, const pisco_bat
     CHK(c->cot.ensure(std::max<long>(nsamples, 1)));
     CHK(c->support.ensure(1));
     HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
-    const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
     const bool det = c->adjmap_det != 0;
-    const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
     const double *scale = nullptr;
-    if (det) {
-        std::vector<const float *> bl;
-        std::vector<long> nl;
-        for (int k = 0; k < b->npad; k++) {
-            bl.push_back(b->host[k].beam);
-            nl.push_back(b->host[k].out < 0 ? 0 : b->host[k].npix);
-        }
-        CHK(det_scale(c, bl.data(), nl.data(), (int)bl.size(), g, nullptr,
-                      2L * b->npairs * nsamples, (long)b->npairs * nsamples, st, &scale));
-    }
+    if (det)
+        CHK(batch_det_scale(c, st, b, nsamples, g, &scale));
     /* one single-pair sky adjoint per pair, each with its own pointing and
      * pa + pa_offset rounded to float (what the forward uses), all adding into
      * one FP64 gradient */
-    for (int k = 0; k < b->npad && nsamples > 0; k++) {
-        const BatchPair &q = b->host[k];
-        if (q.out < 0)
-            continue;
-        const long po = (long)q.out * nsamples;
-        HIPCHK(launch_add_offset(pa + po, q.pa_off, c->cot.p, nsamples, st));
-        PairParams p;
-        CHK(setup_pair(c, nside_sky, nullptr, nullptr, nullptr, q.nside, q.npix, q.beam,
-                       nsamples, ra + po, dec + po, c->cot.p, p));
-        p.g_a = g + 2 * po;
-        p.g_b = g + 2 * po + nsamples;
-        p.gI = c->acc.p;
-        p.gQ = c->acc.p + npix;
-        p.gU = c->acc.p + 2 * npix;
-        if (c->adjmap_support) {
-            HIPCHK(launch_beam_support(q.beam, q.npix, c->support.p, st));
-            p.beam_last = c->support.p;
-        }
-        hipError_t e = launch_adjmap_lds(p, adj_chunk(c, nside_sky, p.rho_max), cap,
-                                         c->adjmap_rw, nb, st, nullptr, 0.0, 1, scale,
-                                         adj_lanes(c, p));
-        if (e != hipSuccess)
-            return fail(PISCO_ERR_DEVICE, "per-pair sky adjoint launch failed: %s",
-                        hipGetErrorString(e));
-    }
+    CHK(batch_adjoint_pairwise(c, st, b, nside_sky, nsamples, ra, dec, pa, nsamples, g, scale));
     float *out[3] = {gI, gQ, gU};
     for (int k = 0; k < 3; k++) {
         if (det)
@@ -1303,6 +1380,20 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     const long npix = 12L * nside_sky * nside_sky;
     CHK(c->acc.ensure(3 * (size_t)npix));
     HIPCHK(launch_zero_f64(c->acc.p, 3 * npix, st));
+    float *out[3] = {gI, gQ, gU};
+    if (pair_major(c, b)) {
+        const double *scale = nullptr;
+        if (c->adjmap_det)
+            CHK(batch_det_scale(c, st, b, nsamples, g, &scale));
+        CHK(batch_adjoint_pairwise(c, st, b, nside_sky, nsamples, ra, dec, pa, 0, g, scale));
+        for (int k = 0; k < 3; k++) {
+            if (scale)
+                HIPCHK(launch_fix_to_f32(c->acc.p + k * npix, scale, out[k], npix, st));
+            else
+                HIPCHK(launch_f64_to_f32(c->acc.p + k * npix, out[k], npix, st));
+        }
+        return PISCO_OK;
+    }
     bp.g = g;
     bp.gI = c->acc.p;
     bp.gQ = c->acc.p + npix;
@@ -1334,22 +1425,13 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     /* the window capacity that fits next to the group's per-wave tables */
     const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, grp, det);
     const double *scale = nullptr;
-    if (det) {
-        std::vector<const float *> bl;
-        std::vector<long> nl;
-        for (int k = 0; k < b->npad; k++) {
-            bl.push_back(b->host[k].beam);
-            nl.push_back(b->host[k].out < 0 ? 0 : b->host[k].npix);
-        }
-        CHK(det_scale(c, bl.data(), nl.data(), (int)bl.size(), g, nullptr,
-                      2L * b->npairs * nsamples, (long)b->npairs * nsamples, st, &scale));
-    }
+    if (det)
+        CHK(batch_det_scale(c, st, b, nsamples, g, &scale));
     hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, &bp,
                                      std::min(b->rho_support, q0.rho_max), grp, scale);
     if (e != hipSuccess)
         return fail(PISCO_ERR_DEVICE, "batched sky adjoint launch failed: %s",
                     hipGetErrorString(e));
-    float *out[3] = {gI, gQ, gU};
     for (int k = 0; k < 3; k++) {
         if (det)
             HIPCHK(launch_fix_to_f32(c->acc.p + k * npix, scale, out[k], npix, st));
