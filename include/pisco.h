@@ -132,7 +132,11 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * 8; 0, the default, picks 8 below 1,000 pixels per disc, else 64),
  * "batch_pair_major" (batched launches: 1 runs the pairs one after another
  * with the single-pair kernels, 0 all pairs per chunk of samples; -1, the
- * default, picks 1 when a pair's packed beam exceeds 4 MiB);
+ * default, picks 1 when a pair's packed beam exceeds 4 MiB),
+ * "batch_pair_range" (batched launches over consecutive ranges of the pair
+ * table, sorted by disc radius: k > 0 (a multiple of 4) pairs per range, -1
+ * one range of all pairs; 0, the default, ranges whose packed beams add up
+ * to at most 4 MiB);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);

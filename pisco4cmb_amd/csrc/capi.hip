@@ -123,6 +123,7 @@ struct pisco_ctx {
     int forward_lanes = 0;        /* lanes per sample of the forward: 64, 32, 16, 8; 0 = auto */
     int adjmap_lanes = 0;         /* lanes per sample of the sky adjoint: 64, 16, 8; 0 = auto */
     int batch_pair_major = -1;    /* batched launches pair by pair: 1 yes, 0 no, -1 auto */
+    int batch_pair_range = 0;     /* pairs per batched launch: 4k; 0 = auto; -1 = all */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
@@ -406,6 +407,11 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 0 && value != 8 && value != 16 && value != 32 && value != 64)
             return fail(PISCO_ERR_INVALID, "forward_lanes must be 0 (auto), 8, 16, 32 or 64");
         c->forward_lanes = (int)value;
+    } else if (k == "batch_pair_range") {
+        if (value < -1 || value > 65536 || (value > 0 && value % 4))
+            return fail(PISCO_ERR_INVALID,
+                        "batch_pair_range must be -1 (all), 0 (auto) or a multiple of 4");
+        c->batch_pair_range = (int)value;
     } else if (k == "batch_pair_major") {
         if (value < -1 || value > 1)
             return fail(PISCO_ERR_INVALID, "batch_pair_major must be -1 (auto), 0 or 1");
@@ -437,6 +443,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjmap_lanes;
     else if (k == "batch_pair_major")
         *value = c->batch_pair_major;
+    else if (k == "batch_pair_range")
+        *value = c->batch_pair_range;
     else if (k == "adjmap_cap")
         *value = c->adjmap_cap;
     else if (k == "adjmap_rw")
@@ -1156,6 +1164,40 @@ int setup_batch(pisco_ctx *c, const pisco_batch *b, int nside_sky, const float *
     return PISCO_OK;
 }
 
+/* The pair ranges of a batched launch: starts of consecutive ranges of the
+ * pair table (sorted by disc radius, largest first), each a multiple of 4
+ * pairs, ending with npad.  The tuning value: -1 one range, k > 0 ranges of k
+ * pairs; 0 (auto): ranges whose packed beams (32 B per row) add up to at most
+ * one XCD's 4 MiB L2, with at least 4 pairs each.  On 32 pairs with beams of
+ * 0.05-1.1 MB, ranges of 8 pairs took the forward + sky adjoint from 134.5 to
+ * 109.1 ms, almost all of it in the adjoint (ranges of 4: 111.1 ms; of 16:
+ * 122.6 ms). */
+std::vector<int> batch_ranges(const pisco_ctx *c, const pisco_batch *b)
+{
+    std::vector<int> cut{0};
+    if (c->batch_pair_range > 0) {
+        for (int r = c->batch_pair_range; r < b->npad; r += c->batch_pair_range)
+            cut.push_back(r);
+    } else if (c->batch_pair_range == 0) {
+        double bytes = 0.0;
+        int start = 0;
+        for (int k = 0; k < b->npad; k += 4) {
+            double blk = 0.0;
+            for (int j = k; j < k + 4 && j < b->npad; j++)
+                if (b->host[j].out >= 0)
+                    blk += 32.0 * (double)b->host[j].npix;
+            if (k > start && bytes + blk > 4.0 * 1024 * 1024) {
+                cut.push_back(k);
+                start = k;
+                bytes = 0.0;
+            }
+            bytes += blk;
+        }
+    }
+    cut.push_back(b->npad);
+    return cut;
+}
+
 /* The deterministic mode's scale for a batch: every pair's beam, the
  * cotangents of all pairs (2 npairs nsamples values), at most npairs
  * nsamples contributions per pixel. */
@@ -1274,7 +1316,6 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
     BatchParams p;
     CHK(setup_batch(c, b, nside_sky, I, Q, U, nsamples, ra, dec, pa, p));
     const int G = c->batch_group;
-    p.ngroups = b->npad / G;
     p.tod = tod;
     hipStream_t st = pick(c, stream);
     CHK(c->sky4.ensure(4 * (size_t)p.hs.npix));
@@ -1282,10 +1323,17 @@ int pisco_dev_batched_forward(pisco_ctx *c, void *stream, const pisco_batch *b, 
     p.sky4 = (const float4 *)c->sky4.p;
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : nsamples;
     chunk = std::max<long>(chunk, std::min<long>(nsamples, 1L << 20));
-    for (long s0 = 0; s0 < nsamples; s0 += chunk) {
-        p.s_begin = s0;
-        p.s_end = std::min(nsamples, s0 + chunk);
-        HIPCHK(launch_batch_forward(G, p, st));
+    const std::vector<int> cut = batch_ranges(c, b);
+    for (size_t r = 0; r + 1 < cut.size(); r++) {   /* ranges of the sorted pair table */
+        const int r0 = cut[r], r1 = cut[r + 1];
+        p.pairs = b->table + r0;
+        p.npairs = r1 - r0;
+        p.ngroups = (r1 - r0) / G;
+        for (long s0 = 0; s0 < nsamples; s0 += chunk) {
+            p.s_begin = s0;
+            p.s_end = std::min(nsamples, s0 + chunk);
+            HIPCHK(launch_batch_forward(G, p, st));
+        }
     }
     return PISCO_OK;
 }
@@ -1398,40 +1446,47 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     bp.gI = c->acc.p;
     bp.gQ = c->acc.p + npix;
     bp.gU = c->acc.p + 2 * npix;
-    /* the union disc (the first, largest pair) drives the window passes */
-    const BatchPair &q0 = b->host[0];
-    PairParams p;
-    memset(&p, 0, sizeof p);
-    p.hs = bp.hs;
-    p.rings = bp.rings;
-    p.hb = hpx_make(q0.nside);
-    p.npix_beam = q0.npix;
-    p.rho_max = q0.rho_max;
-    p.cos_rmax = q0.cos_rmax;
-    p.ra = ra;
-    p.dec = dec;
-    p.pa = pa;
-    p.gI = bp.gI;
-    p.gQ = bp.gQ;
-    p.gU = bp.gU;
-    p.s_begin = 0;
-    p.s_end = nsamples;
     const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
     const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
     const bool det = c->adjmap_det != 0;
     /* the deterministic mode takes one pair per wave (integer windows) */
     const int grp = det ? 1 : c->batch_adj_group;
-    bp.ngroups = b->npad / grp;
     /* the window capacity that fits next to the group's per-wave tables */
     const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, grp, det);
     const double *scale = nullptr;
     if (det)
         CHK(batch_det_scale(c, st, b, nsamples, g, &scale));
-    hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, &bp,
-                                     std::min(b->rho_support, q0.rho_max), grp, scale);
-    if (e != hipSuccess)
-        return fail(PISCO_ERR_DEVICE, "batched sky adjoint launch failed: %s",
-                    hipGetErrorString(e));
+    const std::vector<int> cut = batch_ranges(c, b);
+    for (size_t r = 0; r + 1 < cut.size(); r++) {   /* ranges of the sorted pair table */
+        const int r0 = cut[r], r1 = cut[r + 1];
+        bp.pairs = b->table + r0;
+        bp.npairs = r1 - r0;
+        bp.ngroups = (r1 - r0) / grp;
+        /* the range's union disc (its first, largest pair) drives the window
+         * passes */
+        const BatchPair &q0 = b->host[r0];
+        PairParams p;
+        memset(&p, 0, sizeof p);
+        p.hs = bp.hs;
+        p.rings = bp.rings;
+        p.hb = hpx_make(q0.nside);
+        p.npix_beam = q0.npix;
+        p.rho_max = q0.rho_max;
+        p.cos_rmax = q0.cos_rmax;
+        p.ra = ra;
+        p.dec = dec;
+        p.pa = pa;
+        p.gI = bp.gI;
+        p.gQ = bp.gQ;
+        p.gU = bp.gU;
+        p.s_begin = 0;
+        p.s_end = nsamples;
+        hipError_t e = launch_adjmap_lds(p, chunk, cap, c->adjmap_rw, nb, st, &bp,
+                                         std::min(b->rho_support, q0.rho_max), grp, scale);
+        if (e != hipSuccess)
+            return fail(PISCO_ERR_DEVICE, "batched sky adjoint launch failed: %s",
+                        hipGetErrorString(e));
+    }
     for (int k = 0; k < 3; k++) {
         if (det)
             HIPCHK(launch_fix_to_f32(c->acc.p + k * npix, scale, out[k], npix, st));

@@ -221,3 +221,38 @@ def test_pair_major_auto_for_large_beams(oracle):
     for a, r in zip(got, ref):
         assert rel_rms(a, r) < 1e-6
     batch.close()
+
+
+def test_pair_ranges(case):
+    """Batched launches over ranges of the pair table (batch_pair_range 4 and
+    auto) against one range of all pairs (-1): forward and sky adjoint within
+    relative RMS 1e-6; the deterministic sky adjoint gives the same bits."""
+    sky, pairs, ra, dec, pa = case
+    I, Q, U = (t(x) for x in sky[:3])
+    rat, dect, pat = t(ra), t(dec), t(pa)
+    specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
+    batch = DetectorBatch(specs)
+    ctx = torch_ops._ctx(torch.device("cuda:0"))
+    g = torch.randn((len(pairs), 2, ra.size), device="cuda")
+    res, det = {}, {}
+    try:
+        check(lib.pisco_set_tuning(ctx, b"batch_pair_major", 0))
+        for rng_ in (-1, 4, 0):
+            check(lib.pisco_set_tuning(ctx, b"batch_pair_range", rng_))
+            tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
+            adj = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
+            res[rng_] = (tod, adj)
+            check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 1))
+            det[rng_] = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
+            check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 0))
+    finally:
+        for k, v in ((b"batch_pair_range", 0), (b"batch_pair_major", -1),
+                     (b"adjmap_deterministic", 0)):
+            check(lib.pisco_set_tuning(ctx, k, v))
+    for rng_ in (4, 0):
+        assert rel_rms(res[rng_][0], res[-1][0]) < 1e-6
+        for a, b in zip(res[rng_][1], res[-1][1]):
+            assert rel_rms(a, b) < 1e-6
+        for a, b in zip(det[rng_], det[-1]):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    batch.close()
