@@ -4,9 +4,13 @@ the device and keeps the fastest (pisco_set_tuning).
     best = autotune_batch(batch, I, Q, U, ra, dec, pa, nside_sky)
     best = autotune_pair(I, Q, U, beam_a, beam_b, ra, dec, pa, nside_sky, nside_beam)
 
-Batched candidates: pairs per wave of the batched forward (batch_group 1, 2,
-4) and of the batched sky adjoint (batch_adj_group 1, 2, 4), and samples per
-LDS window of the sky adjoint (adjmap_chunk 8, 16, 32).  Single-pair
+Batched candidates: pair by pair or all pairs per chunk of samples
+(batch_pair_major 0, 1), pairs per launch range (batch_pair_range 0 = by beam
+bytes, 4, 8, 16, -1 = all), pairs per wave of the batched forward
+(batch_group 1, 2, 4) and of the batched sky adjoint (batch_adj_group 1, 2,
+4), and samples per LDS window of the sky adjoint (adjmap_chunk 8, 16, 32).
+Keys are tuned one after the other, each with the best value of the ones
+before.  Single-pair
 candidates: lanes per sample of the forward (forward_lanes 64, 32, 16, 8:
 one to eight samples per wave), lanes per sample of the sky adjoint
 (adjmap_lanes 64, 16, 8) and samples per LDS window (adjmap_chunk 16, 32, 64,
@@ -46,14 +50,19 @@ def _get(ctx, key):
 
 
 def autotune_batch(batch, I, Q, U, ra, dec, pa, nside_sky, sample=65536, reps=2):
-    """Picks batch_group, batch_adj_group and adjmap_chunk for a
-    DetectorBatch on this pointing; returns {key: (best, {value: seconds})}."""
+    """Picks batch_pair_major, batch_pair_range, batch_group, batch_adj_group
+    and adjmap_chunk for a DetectorBatch on this pointing; returns {key:
+    (best, {value: seconds})}."""
     ctx = _ctx(batch.device)
     n = min(sample, ra.numel())
     r, d, p = ra[:n], dec[:n], pa[:n]
     g = torch.randn((batch.npairs, 2, n), device=batch.device)
     result = {}
+    step = lambda: (batch.forward(I, Q, U, r, d, p, nside_sky),
+                    batch.adjoint_map(g, r, d, p, nside_sky))
     for key, values, fn in (
+            ("batch_pair_major", (0, 1), step),
+            ("batch_pair_range", (0, 4, 8, 16, -1), step),
             ("batch_group", (1, 2, 4), lambda: batch.forward(I, Q, U, r, d, p, nside_sky)),
             ("batch_adj_group", (1, 2, 4), lambda: batch.adjoint_map(g, r, d, p, nside_sky)),
             ("adjmap_chunk", (8, 16, 32), lambda: batch.adjoint_map(g, r, d, p, nside_sky))):
@@ -119,4 +128,5 @@ def tuning(device=None):
     ctx = _ctx(device or torch.device("cuda", torch.cuda.current_device()))
     return {k: _get(ctx, k) for k in ("batch_group", "batch_adj_group", "adjmap_chunk",
                                       "adjmap_cap", "adjmap_rw", "adjmap_support",
-                                      "forward_lanes", "adjmap_lanes")}
+                                      "forward_lanes", "adjmap_lanes", "batch_pair_major",
+                                      "batch_pair_range")}

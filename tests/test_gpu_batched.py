@@ -144,7 +144,8 @@ def test_autotuner_picks_a_valid_shape(case):
         assert best in times and cur[key] == best
     from pisco4cmb_amd._lib import check as _check
     ctx = torch_ops._ctx(torch.device("cuda:0"))
-    for k, v in (("batch_group", 4), ("batch_adj_group", 4), ("adjmap_chunk", -1)):
+    for k, v in (("batch_group", 4), ("batch_adj_group", 4), ("adjmap_chunk", -1),
+                 ("batch_pair_major", -1), ("batch_pair_range", 0)):
         _check(lib.pisco_set_tuning(ctx, k.encode(), v))
     batch.close()
 
