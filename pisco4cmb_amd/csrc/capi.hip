@@ -236,9 +236,11 @@ int adj_lanes(const pisco_ctx *c, const PairParams &p)
 }
 
 /* Launches op over all samples in chunks of ptgPerConv (or one launch). */
-int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st)
+int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st, bool sky_packed = false)
 {
-    if (op == OP_FWD && !c->beam_interp) {
+    if (op == OP_FWD && !c->beam_interp && sky_packed) {
+        p.sky4 = (const float4 *)c->sky4.p;   /* packed by the caller on this stream */
+    } else if (op == OP_FWD && !c->beam_interp) {
         /* the nearest-pixel forward reads the sky packed as one float4 per
          * pixel: one load per visit instead of three */
         const long npix = p.hs.npix;
@@ -1241,6 +1243,9 @@ int batch_forward_pairwise(pisco_ctx *c, hipStream_t st, const pisco_batch *b, i
                            float *tod)
 {
     CHK(c->cot.ensure(std::max<long>(nsamples, 1)));
+    const long npix = 12L * nside_sky * nside_sky;
+    CHK(c->sky4.ensure(4 * (size_t)npix));   /* packed once for all pairs */
+    HIPCHK(launch_pack_sky(I, Q, U, npix, (float4 *)c->sky4.p, st));
     for (const BatchPair &q : b->host) {
         if (q.out < 0 || nsamples == 0)
             continue;
@@ -1251,7 +1256,7 @@ int batch_forward_pairwise(pisco_ctx *c, hipStream_t st, const pisco_batch *b, i
                        dec + po, c->cot.p, p));
         p.tod_a = tod + 2L * q.out * nsamples;
         p.tod_b = p.tod_a + nsamples;
-        CHK(run_pair(c, OP_FWD, p, st));
+        CHK(run_pair(c, OP_FWD, p, st, true));
     }
     return PISCO_OK;
 }
