@@ -136,7 +136,10 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * "batch_pair_range" (batched launches over consecutive ranges of the pair
  * table, sorted by disc radius: k > 0 (a multiple of 4) pairs per range, -1
  * one range of all pairs; 0, the default, ranges whose packed beams add up
- * to at most 4 MiB);
+ * to at most 4 MiB), "adjbeam_bands" (beam adjoint: 1, the default, sums in
+ * LDS bands of beam rows over annuli of the discs; 0 global FP64 atomics),
+ * "adjbeam_chunk" (samples per banded work item, default 1024),
+ * "adjbeam_rows" (rows per band; 0, the default, as many as fit the LDS);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);

@@ -124,6 +124,9 @@ struct pisco_ctx {
     int adjmap_lanes = 0;         /* lanes per sample of the sky adjoint: 64, 16, 8; 0 = auto */
     int batch_pair_major = -1;    /* batched launches pair by pair: 1 yes, 0 no, -1 auto */
     int batch_pair_range = 0;     /* pairs per batched launch: 4k; 0 = auto; -1 = all */
+    int adjbeam_bands = 1;        /* beam adjoint in LDS bands (1) or global atomics (0) */
+    int adjbeam_chunk = 1024;     /* samples per work item of the banded beam adjoint */
+    int adjbeam_rows = 0;         /* beam rows per band; 0 = as many as fit the LDS */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
@@ -414,6 +417,18 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
             return fail(PISCO_ERR_INVALID,
                         "batch_pair_range must be -1 (all), 0 (auto) or a multiple of 4");
         c->batch_pair_range = (int)value;
+    } else if (k == "adjbeam_bands") {
+        if (value != 0 && value != 1)
+            return fail(PISCO_ERR_INVALID, "adjbeam_bands must be 0 or 1");
+        c->adjbeam_bands = (int)value;
+    } else if (k == "adjbeam_rows") {
+        if (value < 0 || value > (1L << 30))
+            return fail(PISCO_ERR_INVALID, "adjbeam_rows out of range");
+        c->adjbeam_rows = (int)value;
+    } else if (k == "adjbeam_chunk") {
+        if (value < 1 || value > (1L << 24))
+            return fail(PISCO_ERR_INVALID, "adjbeam_chunk must be in [1, 2^24]");
+        c->adjbeam_chunk = (int)value;
     } else if (k == "batch_pair_major") {
         if (value < -1 || value > 1)
             return fail(PISCO_ERR_INVALID, "batch_pair_major must be -1 (auto), 0 or 1");
@@ -447,6 +462,12 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->batch_pair_major;
     else if (k == "batch_pair_range")
         *value = c->batch_pair_range;
+    else if (k == "adjbeam_bands")
+        *value = c->adjbeam_bands;
+    else if (k == "adjbeam_chunk")
+        *value = c->adjbeam_chunk;
+    else if (k == "adjbeam_rows")
+        *value = c->adjbeam_rows;
     else if (k == "adjmap_cap")
         *value = c->adjmap_cap;
     else if (k == "adjmap_rw")
@@ -931,8 +952,19 @@ int pisco_dev_adjoint_beam(pisco_ctx *c, void *stream, int nside_sky, const floa
     p.g_a = g_a;
     p.g_b = g_b;
     p.gbeam = c->acc.p;
-    if (g_a || g_b)
+    if ((g_a || g_b) && c->adjbeam_bands && !c->beam_interp) {
+        const long npix = p.hs.npix;
+        CHK(c->sky4.ensure(4 * (size_t)npix));
+        HIPCHK(launch_pack_sky(I, Q, U, npix, (float4 *)c->sky4.p, st));
+        p.sky4 = (const float4 *)c->sky4.p;
+        const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+        hipError_t e = launch_adjbeam_bands(p, c->adjbeam_chunk, c->adjbeam_rows, nb, st);
+        if (e != hipSuccess)
+            return fail(PISCO_ERR_DEVICE, "beam adjoint launch failed: %s",
+                        hipGetErrorString(e));
+    } else if (g_a || g_b) {
         CHK(run_pair(c, OP_ADJ_BEAM, p, st));
+    }
     HIPCHK(launch_f64_to_f32(c->acc.p, gbeam8, 8 * npix_beam, st));
     return PISCO_OK;
 }

@@ -404,6 +404,71 @@ __host__ __device__ inline int disc_ring_span(const Hpx &h, const Disc &d, int r
     return 1;
 }
 
+/* Pixel runs of ring r whose centres lie in the annulus dout minus din (two
+ * discs about the same centre, din the smaller; inner == false: no inner
+ * disc): up to two arcs, each split at the ring's first pixel, so up to 4
+ * runs [p0[k], p0[k] + len[k]).  Returns the number of runs.  Both spans come
+ * from disc_ring_span about the same centre, so the inner span lies inside
+ * the outer one in the same unwrapped positions. */
+__host__ __device__ inline int annulus_ring(const Hpx &h, const Disc &dout, const Disc &din,
+                                            bool inner, int r, int p0[4], int len[4])
+{
+    int lo_o = 0, hi_o = -1, ip1 = 0, nr = 0;
+    const int kind_o = disc_ring_span(h, dout, r, lo_o, hi_o, ip1, nr);
+    if (kind_o == 0)
+        return 0;
+    int kind_i = 0, lo_i = 0, hi_i = -1;
+    if (inner && r >= din.r_first && r <= din.r_last) {
+        int ip1b, nrb;
+        kind_i = disc_ring_span(h, din, r, lo_i, hi_i, ip1b, nrb);
+    }
+    if (kind_i == 2)
+        return 0;   /* the whole ring lies inside the inner disc */
+    int ia[2], ib[2], ni = 0;
+    if (kind_i == 0) {
+        ia[0] = kind_o == 2 ? 0 : lo_o;
+        ib[0] = kind_o == 2 ? nr - 1 : hi_o;
+        ni = 1;
+    } else if (kind_o == 2) {   /* the ring minus the inner span */
+        ia[0] = hi_i + 1;
+        ib[0] = lo_i - 1 + nr;
+        ni = 1;
+    } else {
+        if (lo_i > lo_o) {
+            ia[ni] = lo_o;
+            ib[ni] = lo_i - 1;
+            ni++;
+        }
+        if (hi_o > hi_i) {
+            ia[ni] = hi_i + 1;
+            ib[ni] = hi_o;
+            ni++;
+        }
+    }
+    int n = 0;
+    for (int k = 0; k < ni; k++) {
+        const int l = ib[k] - ia[k] + 1;   /* 1 .. nr */
+        if (l <= 0)
+            continue;
+        int st = ia[k] % nr;
+        if (st < 0)
+            st += nr;
+        if (st + l <= nr) {
+            p0[n] = ip1 + st;
+            len[n] = l;
+            n++;
+        } else {
+            p0[n] = ip1 + st;
+            len[n] = nr - st;
+            n++;
+            p0[n] = ip1;
+            len[n] = l - (nr - st);
+            n++;
+        }
+    }
+    return n;
+}
+
 /* ---- the beam-frame mapping of one visit -------------------------------- */
 
 /* Per-sample constants of the rotation (sphtrigo.cpp:29-30 evaluated once

@@ -1585,6 +1585,198 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     }
 }
 
+/* ---- beam adjoint in LDS bands --------------------------------------------
+ *
+ * The beam adjoint adds six values per visit to the visit's beam row.  With
+ * global FP64 atomics that is bound by the atomic rate (every sample's disc
+ * touches the whole beam support, so no short span of samples repeats a
+ * row).  Here the beam rows are split into bands of `rows` consecutive rows
+ * that fit the LDS; a work item is (band, chunk of samples).  A point in a
+ * pixel of beam ring R lies between the centres of rings R-1 and R+1, so the
+ * visits whose beam pixel falls in rows [lo, hi) of rings Rf..Rl have
+ * rho in [theta(Rf-1), theta(Rl+1)]: the item walks only that annulus of each
+ * sample's disc (annulus_ring), keeps the visits whose exact beam pixel is in
+ * the band, sums their values in float in LDS and adds the band to the FP64
+ * gradient once per item. */
+constexpr int kBandRuns = 4 * kWaveSize;   /* <= 4 runs per ring, 64 rings per pass */
+
+struct BandRunTable {
+    int pref[kBandRuns + 1];
+    int pix0[kBandRuns];
+    int ring[kBandRuns];
+};
+
+struct BeamBands {
+    int rows;      /* rows per band */
+    int nband;
+    int chunk;     /* samples per work item */
+    long nchunks;
+};
+
+__device__ inline int build_band_runs(const Hpx &hs, const Disc &dout, const Disc &din,
+                                      bool inner, int r0, int lane, BandRunTable &t)
+{
+    const int r = r0 + lane;
+    int p0[4] = {0, 0, 0, 0}, ln[4] = {0, 0, 0, 0};
+    int ns = 0;
+    if (r <= dout.r_last)
+        ns = annulus_ring(hs, dout, din, inner, r, p0, ln);
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        tot += k < ns ? ln[k] : 0;
+    const int cnt_in = wave_incl_scan(ns, lane);
+    const int tot_in = wave_incl_scan(tot, lane);
+    const int cbase = cnt_in - ns;
+    int off = tot_in - tot;
+    for (int k = 0; k < ns; k++) {
+        t.pix0[cbase + k] = p0[k];
+        t.ring[cbase + k] = r;
+        t.pref[cbase + k] = off;
+        off += ln[k];
+    }
+    const int nrun = __shfl(cnt_in, kWaveSize - 1, kWaveSize);
+    const int total = __shfl(tot_in, kWaveSize - 1, kWaveSize);
+    if (lane == 0)
+        t.pref[nrun] = total;
+    wave_sync();
+    return total;
+}
+
+/* The walk of one annulus pass: FP64 certified mapping, software pipelined
+ * as walk_visits; every load of the loop issued unconditionally.  use(q) is
+ * called for each visit whose beam pixel lies in rows [lo, lo + nrow). */
+template <typename Use>
+__device__ inline void walk_band(const PairParams &p, const BandRunTable &t, int total, int lane,
+                                 const Centre &c, double c2psi, double s2psi, int lo, int nrow,
+                                 Use &use)
+{
+    if (total <= 0)
+        return;
+    int s = 0;
+    int v = lane;
+    if (v < total)
+        while (v >= t.pref[s + 1])
+            s++;
+    RingRec rr = p.rings[t.ring[s]];
+    __builtin_amdgcn_s_waitcnt(0);   /* as in walk_visits */
+    Pending q;
+    q.live = false;
+    for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
+        const bool act = v < total;
+        const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
+        const int r = t.ring[s];
+        const RingRec cur = rr;
+        const int vn = v + kWaveSize;
+        if (vn < total)
+            while (vn >= t.pref[s + 1])
+                s++;
+        rr = p.rings[t.ring[s]];
+        float c2, s2;
+        const int bp = beam_pixel_f64(p.hb, p.fc64, c, c2psi, s2psi,
+                                      ring_phi(p.hs, cur, r, pix), cur.cdp, cur.sdp, c2, s2);
+        if (q.live)
+            use(q);
+        q.live = act && (unsigned)(bp - lo) < (unsigned)nrow;
+        q.pix = pix;
+        q.bp = bp;
+        q.c2 = c2;
+        q.s2 = s2;
+        const float4 sv = p.sky4[q.live ? pix : 0];
+        q.sI = sv.x;
+        q.sQ = sv.y;
+        q.sU = sv.z;
+    }
+    if (q.live)
+        use(q);
+}
+
+__global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band(PairParams p, BeamBands B)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    BandRunTable *tabs = (BandRunTable *)smem;
+    float *acc = (float *)(tabs + kAdjWaves);
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWaveSize);
+    const int lane = tid % kWaveSize;
+    BandRunTable &t = tabs[wave];
+    const long nitems = B.nchunks * B.nband;
+    for (long it = xcd_remap(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
+        /* consecutive items: the bands of one chunk (its samples stay in L2) */
+        const long ch = it / B.nband;
+        const int k = (int)(it - ch * B.nband);
+        const int lo = k * B.rows;
+        const int nrow = min(p.npix_beam - lo, B.rows);
+        /* the annulus of rows [lo, lo + nrow) */
+        const int rf = pix2ring(p.hb, lo), rl = pix2ring(p.hb, lo + nrow - 1);
+        const bool inner = rf > 1;
+        const double r_in = inner ? ring_theta(p.hb, rf - 1) - 1e-9 : 0.0;
+        double r_out = rl + 1 <= 4 * p.hb.nside - 1 ? ring_theta(p.hb, rl + 1) + 1e-9 : p.rho_max;
+        double c_out;
+        if (r_out >= p.rho_max) {
+            r_out = p.rho_max;
+            c_out = p.cos_rmax;
+        } else {
+            c_out = pl_cos(r_out);
+        }
+        const double c_in = pl_cos(r_in);
+        for (int i = tid; i < 6 * nrow; i += kAdjBlock)
+            acc[i] = 0.f;
+        __syncthreads();
+        const long s0 = p.s_begin + ch * B.chunk;
+        const long s1 = s0 + B.chunk < p.s_end ? s0 + B.chunk : p.s_end;
+        for (long i = s0 + wave; i < s1; i += kAdjWaves) {
+            const float ga = unif(p.g_a ? p.g_a[i] : 0.f);
+            const float gb = unif(p.g_b ? p.g_b[i] : 0.f);
+            if (ga == 0.f && gb == 0.f)
+                continue;   /* adds nothing (wave-uniform) */
+            Centre c = centre_make(p.ra[i], p.dec[i], p.pa[i]);
+            c.ra = uni(c.ra);
+            c.dec = uni(c.dec);
+            c.psi = uni(c.psi);
+            c.cdc = uni(c.cdc);
+            c.sdc = uni(c.sdc);
+            double s2psi, c2psi;
+            pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+            c2psi = uni(c2psi);
+            s2psi = uni(s2psi);
+            Disc dout, din;
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, r_out, c_out, dout);
+            if (inner)
+                disc_setup(p.hs, kHalfPi - c.dec, c.ra, r_in, c_in, din);
+            else
+                din = dout;   /* not read */
+            auto use = [&](const Pending &q) {
+                const float X = q.sQ * q.c2 + q.sU * q.s2;
+                const float Y = -q.sQ * q.s2 + q.sU * q.c2;
+                float *a = acc + 6 * (q.bp - lo);
+                atomicAdd(&a[0], ga * q.sI);
+                atomicAdd(&a[1], ga * X);
+                atomicAdd(&a[2], ga * Y);
+                atomicAdd(&a[3], gb * q.sI);
+                atomicAdd(&a[4], -(gb * X));
+                atomicAdd(&a[5], -(gb * Y));
+            };
+            for (int r0 = dout.r_first; r0 <= dout.r_last; r0 += kWaveSize) {
+                const int total = build_band_runs(p.hs, dout, din, inner, r0, lane, t);
+                walk_band(p, t, total, lane, c, c2psi, s2psi, lo, nrow, use);
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        /* add the band to the FP64 gradient */
+        for (int rr = tid; rr < nrow; rr += kAdjBlock) {
+            const float *a = acc + 6 * rr;
+            double *g = p.gbeam + 8 * (long)(lo + rr);
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+                if (a[m] != 0.f)
+                    unsafeAtomicAdd(&g[m], (double)a[m]);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_fill_i32(int *x, int v) { *x = v; }
 
 /* Last row with a non-zero value in columns 0..5 (wave max, one atomic per
@@ -2023,6 +2215,49 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
     case 4: return launch_adjmap_bg<4, false>(p, L, bytes, g, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+int adjbeam_band_rows()
+{
+    return (int)((163840 - sizeof(BandRunTable) * kAdjWaves) / (6 * sizeof(float)));
+}
+
+hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks,
+                                hipStream_t st)
+{
+    const long n = p.s_end - p.s_begin;
+    if (n <= 0 || p.npix_beam <= 0)
+        return hipSuccess;
+    if (chunk < 1)
+        return hipErrorInvalidValue;
+    BeamBands B;
+    B.rows = adjbeam_band_rows();
+    if (rows > 0 && rows < B.rows)
+        B.rows = rows;
+    B.nband = (p.npix_beam + B.rows - 1) / B.rows;
+    /* smaller items when there are too few to give every block two */
+    const long fill = n * B.nband / (2L * std::max(nblocks, 1));
+    B.chunk = (int)std::max<long>(std::min<long>(chunk, std::max<long>(fill, 64)), 1);
+    B.nchunks = (n + B.chunk - 1) / B.chunk;
+    const size_t bytes = sizeof(BandRunTable) * kAdjWaves + 6 * sizeof(float) * (size_t)B.rows;
+    constexpr int kDevs = 64;
+    static std::atomic<bool> configured[kDevs];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= kDevs || !configured[dev].load()) {
+        e = hipFuncSetAttribute((const void *)k_adjbeam_band,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess)
+            return e;
+        if (dev >= 0 && dev < kDevs)
+            configured[dev].store(true);
+    }
+    const long items = B.nchunks * B.nband;
+    const long g = std::min<long>(items, nblocks);
+    hipLaunchKernelGGL(k_adjbeam_band, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, B);
+    return hipGetLastError();
 }
 
 int adjmap_describe(char *buf, size_t n, int cap, int rw)

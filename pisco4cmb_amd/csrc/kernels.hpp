@@ -140,6 +140,12 @@ hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st);
 /* Forward with 64/sub samples per wave, sub = 16 or 32 lanes each (small
  * discs); same results as launch_pair(OP_FWD). */
 hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st);
+/* Beam adjoint in LDS bands of beam rows (work items: band x chunk of
+ * samples); p.gbeam accumulates in FP64, p.sky4 holds the packed sky.  rows:
+ * rows per band (0 or more than fits: as many as fit the LDS). */
+hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks,
+                                hipStream_t st);
+int adjbeam_band_rows();
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
 /* fx_scale (device, or NULL): deterministic mode with the scale of
