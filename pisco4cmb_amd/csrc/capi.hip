@@ -285,21 +285,25 @@ int det_scale(pisco_ctx *c, const float *const *beams, const long *npix, int nbe
 }
 
 /* Samples per LDS window of a single-pair sky adjoint over discs of radius
- * rho: the tuning value, or (auto) the largest power of two in [16, 128]
- * that keeps a window at about 96,000 expected visits.  Each window costs
- * three block-wide passes besides its visits; at 6,000 pixels per disc
- * 16 samples per window measured fastest, at 135 pixels 128 samples took
- * 0.71x the time of 16. */
-int adj_chunk(const pisco_ctx *c, int nside_sky, double rho)
+ * rho with `lanes` lanes per sample: the tuning value, or (auto) the largest
+ * power of two in [16, 512] that keeps a window at about 96,000 expected
+ * visits, and at least one sample per lane group of every wave of the block
+ * (16 x 64 / lanes).  Each window costs three block-wide passes besides its
+ * visits; at 6,000 pixels per disc 16 samples per window measured fastest,
+ * at 135 pixels 256 samples took 0.64x the time of 16 (one sample per wave)
+ * and, with 8 lanes per sample, 0.97x the time of 128; at 540 pixels with 8
+ * lanes, 32 samples took 2.0x the time of 128. */
+int adj_chunk(const pisco_ctx *c, int nside_sky, double rho, int lanes)
 {
     if (c->adjmap_chunk > 0)
         return c->adjmap_chunk;
     const double s = pl_sin(0.5 * std::min(rho, kPi));
     const double disc = 12.0 * nside_sky * nside_sky * s * s;   /* npix (1 - cos rho) / 2 */
     int chunk = 16;
-    while (chunk < 128 && 2.0 * chunk * disc <= 96000.0)
+    while (chunk < 512 && 2.0 * chunk * disc <= 96000.0)
         chunk *= 2;
-    return chunk;
+    const int fill = 16 * (64 / std::max(lanes, 1));   /* lane groups of a 16-wave block */
+    return std::max(chunk, std::min(fill, 512));
 }
 
 /* The window capacity that fits the LDS next to the per-wave tables. */
@@ -902,7 +906,7 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
     if (g_a || g_b) {
         if ((c->adjmap_chunk != 0 || det) && !c->beam_interp) {
             int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
-            const int chunk = adj_chunk(c, nside_sky, p.rho_max);
+            const int chunk = adj_chunk(c, nside_sky, p.rho_max, adj_lanes(c, p));
             const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
             if (c->adjmap_support) {
                 CHK(c->support.ensure(1));
@@ -1323,9 +1327,9 @@ int batch_adjoint_pairwise(pisco_ctx *c, hipStream_t st, const pisco_batch *b, i
             HIPCHK(launch_beam_support(q.beam, q.npix, c->support.p, st));
             p.beam_last = c->support.p;
         }
-        hipError_t e = launch_adjmap_lds(p, adj_chunk(c, nside_sky, p.rho_max), cap,
-                                         c->adjmap_rw, nb, st, nullptr, 0.0, 1, scale,
-                                         adj_lanes(c, p));
+        const int lanes = adj_lanes(c, p);
+        hipError_t e = launch_adjmap_lds(p, adj_chunk(c, nside_sky, p.rho_max, lanes), cap,
+                                         c->adjmap_rw, nb, st, nullptr, 0.0, 1, scale, lanes);
         if (e != hipSuccess)
             return fail(PISCO_ERR_DEVICE, "per-pair sky adjoint launch failed: %s",
                         hipGetErrorString(e));
