@@ -119,8 +119,8 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * not depend on them).  Keys: "adjmap_chunk" (samples per LDS window of the sky adjoint;
  * 0 selects direct global atomics; -1, the default, picks 16 to 128 from the
  * disc size for single-pair adjoints and 16 for batched ones), "adjmap_cap" (window pixels),
- * "adjmap_rw" (window rings), "adjmap_blocks" (persistent blocks; 0 = one
- * per CU), "adjmap_support" (1: windows cover only the discs of the beam's
+ * "adjmap_rw" (window rings), "adjmap_blocks" (blocks of the adjoint launches,
+ * each looping over windows; 0, the default, one block per window), "adjmap_support" (1: windows cover only the discs of the beam's
  * non-zero rows, 0: the full rho_max discs; results are the same either way),
  * "batch_group" (batched forward: 1 = one wave per (pair, sample), the
  * single-pair computation; 2 or 4 = pairs per wave sharing each visit's

@@ -114,7 +114,7 @@ struct pisco_ctx {
     int adjmap_chunk = -1;        /* samples per LDS window; 0 = direct atomics, -1 = auto */
     int adjmap_cap = 6144;        /* LDS window pixels */
     int adjmap_rw = 1024;         /* LDS window rings */
-    int adjmap_blocks = 0;        /* persistent blocks; 0 = one per CU */
+    int adjmap_blocks = 0;        /* persistent blocks; 0 = one block per window (work item) */
     int adjmap_support = 1;       /* size LDS windows by the beam's non-zero rows */
     int batch_group = 4;          /* pairs per wave in batched forwards (1: per-pair items) */
     int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
@@ -905,7 +905,7 @@ int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_b
     }
     if (g_a || g_b) {
         if ((c->adjmap_chunk != 0 || det) && !c->beam_interp) {
-            int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+            const int nb = c->adjmap_blocks;   /* 0: one block per window */
             const int chunk = adj_chunk(c, nside_sky, p.rho_max, adj_lanes(c, p));
             const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, det);
             if (c->adjmap_support) {
@@ -961,8 +961,8 @@ int pisco_dev_adjoint_beam(pisco_ctx *c, void *stream, int nside_sky, const floa
         CHK(c->sky4.ensure(4 * (size_t)npix));
         HIPCHK(launch_pack_sky(I, Q, U, npix, (float4 *)c->sky4.p, st));
         p.sky4 = (const float4 *)c->sky4.p;
-        const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
-        hipError_t e = launch_adjbeam_bands(p, c->adjbeam_chunk, c->adjbeam_rows, nb, st);
+        hipError_t e = launch_adjbeam_bands(p, c->adjbeam_chunk, c->adjbeam_rows,
+                                            c->adjmap_blocks, c->ncu, st);
         if (e != hipSuccess)
             return fail(PISCO_ERR_DEVICE, "beam adjoint launch failed: %s",
                         hipGetErrorString(e));
@@ -1306,7 +1306,7 @@ int batch_adjoint_pairwise(pisco_ctx *c, hipStream_t st, const pisco_batch *b, i
                            long ptg_stride, const float *g, const double *scale)
 {
     const long npix = 12L * nside_sky * nside_sky;
-    const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+    const int nb = c->adjmap_blocks;   /* 0: one block per window */
     const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, 0, scale != nullptr);
     CHK(c->cot.ensure(std::max<long>(nsamples, 1)));
     CHK(c->support.ensure(1));
@@ -1487,7 +1487,7 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     bp.gI = c->acc.p;
     bp.gQ = c->acc.p + npix;
     bp.gU = c->acc.p + 2 * npix;
-    const int nb = c->adjmap_blocks > 0 ? c->adjmap_blocks : c->ncu;
+    const int nb = c->adjmap_blocks;   /* 0: one block per window */
     const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
     const bool det = c->adjmap_det != 0;
     /* the deterministic mode takes one pair per wave (integer windows) */

@@ -2193,7 +2193,11 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
         return hipErrorInvalidValue;   /* deterministic mode: one pair per wave */
     L.fx_scale = fx_scale;
     const size_t bytes = adjmap_lds_bytes(cap, rw, bg, det);
-    const long g = std::min<long>(L.nchunks, nblocks);
+    /* nblocks 0: one block per window.  Blocks start in index order, so the
+     * windows in flight on an XCD stay consecutive in the scan; persistent
+     * blocks (one per CU) drift apart over the launch, and measured 10%
+     * slower on the 5-degree, NSIDE 512 sky adjoint. */
+    const long g = nblocks > 0 ? std::min<long>(L.nchunks, nblocks) : L.nchunks;
     if (bg == 0 && lanes != kWaveSize) {   /* several samples per wave */
         switch (lanes) {
         case 8:
@@ -2222,7 +2226,7 @@ int adjbeam_band_rows()
     return (int)((163840 - sizeof(BandRunTable) * kAdjWaves) / (6 * sizeof(float)));
 }
 
-hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks,
+hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks, int ncu,
                                 hipStream_t st)
 {
     const long n = p.s_end - p.s_begin;
@@ -2236,7 +2240,7 @@ hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nb
         B.rows = rows;
     B.nband = (p.npix_beam + B.rows - 1) / B.rows;
     /* smaller items when there are too few to give every block two */
-    const long fill = n * B.nband / (2L * std::max(nblocks, 1));
+    const long fill = n * B.nband / (2L * std::max(ncu, 1));
     B.chunk = (int)std::max<long>(std::min<long>(chunk, std::max<long>(fill, 64)), 1);
     B.nchunks = (n + B.chunk - 1) / B.chunk;
     const size_t bytes = sizeof(BandRunTable) * kAdjWaves + 6 * sizeof(float) * (size_t)B.rows;
@@ -2255,7 +2259,7 @@ hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nb
             configured[dev].store(true);
     }
     const long items = B.nchunks * B.nband;
-    const long g = std::min<long>(items, nblocks);
+    const long g = nblocks > 0 ? std::min<long>(items, nblocks) : items;   /* 0: one per item */
     hipLaunchKernelGGL(k_adjbeam_band, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, B);
     return hipGetLastError();
 }
