@@ -138,6 +138,8 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * one range of all pairs; 0, the default, ranges whose packed beams add up
  * to at most 4 MiB), "adjbeam_bands" (beam adjoint: 1, the default, sums in
  * LDS bands of beam rows over annuli of the discs; 0 global FP64 atomics),
+ * "adjbeam_lanes" (banded beam adjoint: 8, the default, lanes per sample
+ * with 8 samples per wave, or 64),
  * "adjbeam_chunk" (samples per banded work item, default 1024),
  * "adjbeam_rows" (rows per band; 0, the default, as many as fit the LDS);
  * pisco_get_tuning also reads "cu_count". */

@@ -127,6 +127,7 @@ struct pisco_ctx {
     int adjbeam_bands = 1;        /* beam adjoint in LDS bands (1) or global atomics (0) */
     int adjbeam_chunk = 1024;     /* samples per work item of the banded beam adjoint */
     int adjbeam_rows = 0;         /* beam rows per band; 0 = as many as fit the LDS */
+    int adjbeam_lanes = 8;        /* lanes per sample of the banded beam adjoint: 8 or 64 */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
@@ -425,6 +426,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value != 0 && value != 1)
             return fail(PISCO_ERR_INVALID, "adjbeam_bands must be 0 or 1");
         c->adjbeam_bands = (int)value;
+    } else if (k == "adjbeam_lanes") {
+        if (value != 8 && value != 64)
+            return fail(PISCO_ERR_INVALID, "adjbeam_lanes must be 8 or 64");
+        c->adjbeam_lanes = (int)value;
     } else if (k == "adjbeam_rows") {
         if (value < 0 || value > (1L << 30))
             return fail(PISCO_ERR_INVALID, "adjbeam_rows out of range");
@@ -472,6 +477,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjbeam_chunk;
     else if (k == "adjbeam_rows")
         *value = c->adjbeam_rows;
+    else if (k == "adjbeam_lanes")
+        *value = c->adjbeam_lanes;
     else if (k == "adjmap_cap")
         *value = c->adjmap_cap;
     else if (k == "adjmap_rw")
@@ -962,7 +969,7 @@ int pisco_dev_adjoint_beam(pisco_ctx *c, void *stream, int nside_sky, const floa
         HIPCHK(launch_pack_sky(I, Q, U, npix, (float4 *)c->sky4.p, st));
         p.sky4 = (const float4 *)c->sky4.p;
         hipError_t e = launch_adjbeam_bands(p, c->adjbeam_chunk, c->adjbeam_rows,
-                                            c->adjmap_blocks, c->ncu, st);
+                                            c->adjmap_blocks, c->ncu, c->adjbeam_lanes, st);
         if (e != hipSuccess)
             return fail(PISCO_ERR_DEVICE, "beam adjoint launch failed: %s",
                         hipGetErrorString(e));

@@ -1604,6 +1604,7 @@ struct BandRunTable {
     int pref[kBandRuns + 1];
     int pix0[kBandRuns];
     int ring[kBandRuns];
+    int spare[7];   /* room for eight 8-lane tables (SubBandTable<8>) in the same slot */
 };
 
 struct BeamBands {
@@ -1689,6 +1690,192 @@ __device__ inline void walk_band(const PairParams &p, const BandRunTable &t, int
     }
     if (q.live)
         use(q);
+}
+
+/* Several samples per wave (64/SUB, SUB lanes each) for the band walks:
+ * per (band, sample) the annulus holds few visits and its set-up is a long
+ * dependent chain, so lane groups keep several chains in flight. */
+template <int SUB>
+struct SubBandTable {
+    int pref[4 * SUB + 1];
+    int pix0[4 * SUB];
+    int ring[4 * SUB];
+};
+
+template <int SUB>
+__device__ inline int build_band_sub(const Hpx &hs, const Disc &dout, const Disc &din, bool inner,
+                                     int nrings, int ps, int sl, SubBandTable<SUB> &t)
+{
+    const int r = dout.r_first + ps * SUB + sl;
+    int p0[4] = {0, 0, 0, 0}, ln[4] = {0, 0, 0, 0};
+    int ns = 0;
+    if (ps * SUB + sl < nrings)
+        ns = annulus_ring(hs, dout, din, inner, r, p0, ln);
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        tot += k < ns ? ln[k] : 0;
+    const int cnt_in = seg_incl_scan<SUB>(ns, sl);
+    const int tot_in = seg_incl_scan<SUB>(tot, sl);
+    const int cbase = cnt_in - ns;
+    int off = tot_in - tot;
+    for (int k = 0; k < ns; k++) {
+        t.pix0[cbase + k] = p0[k];
+        t.ring[cbase + k] = r;
+        t.pref[cbase + k] = off;
+        off += ln[k];
+    }
+    const int nrun = __shfl(cnt_in, SUB - 1, SUB);
+    const int total = __shfl(tot_in, SUB - 1, SUB);
+    if (sl == 0)
+        t.pref[nrun] = total;
+    wave_sync();
+    return total;
+}
+
+/* walk_band for a lane group: its `total` visits SUB at a time, for wtot
+ * (the wave's largest total) iterations' worth; a lane without a visit
+ * computes the first visit of its run (or pixel 0 of ring 1) and discards
+ * it. */
+template <int SUB, typename Use>
+__device__ inline void walk_band_sub(const PairParams &p, const SubBandTable<SUB> &t, int total,
+                                     int wtot, int sl, const Centre &c, double c2psi,
+                                     double s2psi, int lo, int nrow, Use &use)
+{
+    if (wtot <= 0)
+        return;
+    const bool any = total > 0;
+    int s = 0;
+    int v = sl;
+    if (v < total)
+        while (v >= t.pref[s + 1])
+            s++;
+    RingRec rr = p.rings[any ? t.ring[s] : 1];
+    __builtin_amdgcn_s_waitcnt(0);   /* as in walk_visits */
+    Pending q;
+    q.live = false;
+    for (int v0 = 0; v0 < wtot; v0 += SUB, v += SUB) {
+        const bool act = v < total;
+        const int r = any ? t.ring[s] : 1;
+        const int pix = any ? t.pix0[s] + (act ? v - t.pref[s] : 0) : 0;
+        const RingRec cur = rr;
+        const int vn = v + SUB;
+        if (vn < total)
+            while (vn >= t.pref[s + 1])
+                s++;
+        rr = p.rings[any ? t.ring[s] : 1];
+        float c2, s2;
+        const int bp = beam_pixel_f64(p.hb, p.fc64, c, c2psi, s2psi,
+                                      ring_phi(p.hs, cur, r, pix), cur.cdp, cur.sdp, c2, s2);
+        if (q.live)
+            use(q);
+        q.live = act && (unsigned)(bp - lo) < (unsigned)nrow;
+        q.pix = pix;
+        q.bp = bp;
+        q.c2 = c2;
+        q.s2 = s2;
+        const float4 sv = p.sky4[q.live ? pix : 0];
+        q.sI = sv.x;
+        q.sQ = sv.y;
+        q.sU = sv.z;
+    }
+    if (q.live)
+        use(q);
+}
+
+/* Band of rows [lo, lo + nrow) of the beam map: the annulus radii. */
+__device__ inline void band_radii(const PairParams &p, int lo, int nrow, bool &inner, double &r_in,
+                                  double &c_in, double &r_out, double &c_out)
+{
+    const int rf = pix2ring(p.hb, lo), rl = pix2ring(p.hb, lo + nrow - 1);
+    inner = rf > 1;
+    r_in = inner ? ring_theta(p.hb, rf - 1) - 1e-9 : 0.0;
+    r_out = rl + 1 <= 4 * p.hb.nside - 1 ? ring_theta(p.hb, rl + 1) + 1e-9 : p.rho_max;
+    if (r_out >= p.rho_max) {
+        r_out = p.rho_max;
+        c_out = p.cos_rmax;
+    } else {
+        c_out = pl_cos(r_out);
+    }
+    c_in = pl_cos(r_in);
+}
+
+template <int SUB>
+__global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band_sub(PairParams p, BeamBands B)
+{
+    constexpr int G = kWaveSize / SUB;
+    static_assert(sizeof(SubBandTable<SUB>) * G <= sizeof(BandRunTable),
+                  "lane-group tables within the wave's run table");
+    extern __shared__ __align__(16) unsigned char smem[];
+    BandRunTable *tabs = (BandRunTable *)smem;
+    float *acc = (float *)(tabs + kAdjWaves);
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWaveSize);
+    const int lane = tid % kWaveSize;
+    const int grp = lane / SUB, sl = lane % SUB;
+    SubBandTable<SUB> &t = ((SubBandTable<SUB> *)&tabs[wave])[grp];
+    const long nitems = B.nchunks * B.nband;
+    for (long it = xcd_remap(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
+        const long ch = it / B.nband;
+        const int k = (int)(it - ch * B.nband);
+        const int lo = k * B.rows;
+        const int nrow = min(p.npix_beam - lo, B.rows);
+        bool inner;
+        double r_in, c_in, r_out, c_out;
+        band_radii(p, lo, nrow, inner, r_in, c_in, r_out, c_out);
+        for (int i = tid; i < 6 * nrow; i += kAdjBlock)
+            acc[i] = 0.f;
+        __syncthreads();
+        const long s0 = p.s_begin + ch * B.chunk;
+        const long s1 = s0 + B.chunk < p.s_end ? s0 + B.chunk : p.s_end;
+        for (long j0 = s0 + (long)wave * G; j0 < s1; j0 += (long)kAdjWaves * G) {
+            const long i = j0 + grp;
+            const bool valid = i < s1;
+            const long iv = valid ? i : j0;
+            const float ga = valid && p.g_a ? p.g_a[iv] : 0.f;
+            const float gb = valid && p.g_b ? p.g_b[iv] : 0.f;
+            const Centre c = centre_make(p.ra[iv], p.dec[iv], p.pa[iv]);
+            double s2psi, c2psi;
+            pl_sincos(2.0 * c.psi, &s2psi, &c2psi);
+            Disc dout, din;
+            disc_setup(p.hs, kHalfPi - c.dec, c.ra, r_out, c_out, dout);
+            if (inner)
+                disc_setup(p.hs, kHalfPi - c.dec, c.ra, r_in, c_in, din);
+            else
+                din = dout;   /* not read */
+            const bool work = valid && (ga != 0.f || gb != 0.f);
+            const int nrings = work && dout.r_last >= dout.r_first ? dout.r_last - dout.r_first + 1
+                                                                    : 0;
+            auto use = [&](const Pending &q) {
+                const float X = q.sQ * q.c2 + q.sU * q.s2;
+                const float Y = -q.sQ * q.s2 + q.sU * q.c2;
+                float *a = acc + 6 * (q.bp - lo);
+                atomicAdd(&a[0], ga * q.sI);
+                atomicAdd(&a[1], ga * X);
+                atomicAdd(&a[2], ga * Y);
+                atomicAdd(&a[3], gb * q.sI);
+                atomicAdd(&a[4], -(gb * X));
+                atomicAdd(&a[5], -(gb * Y));
+            };
+            const int npass = (wave_max_uniform(nrings) + SUB - 1) / SUB;
+            for (int ps = 0; ps < npass; ps++) {
+                const int total = build_band_sub<SUB>(p.hs, dout, din, inner, nrings, ps, sl, t);
+                walk_band_sub<SUB>(p, t, total, wave_max_uniform(total), sl, c, c2psi, s2psi, lo,
+                                   nrow, use);
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        for (int rr = tid; rr < nrow; rr += kAdjBlock) {
+            const float *a = acc + 6 * rr;
+            double *g = p.gbeam + 8 * (long)(lo + rr);
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+                if (a[m] != 0.f)
+                    unsafeAtomicAdd(&g[m], (double)a[m]);
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band(PairParams p, BeamBands B)
@@ -2227,7 +2414,7 @@ int adjbeam_band_rows()
 }
 
 hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks, int ncu,
-                                hipStream_t st)
+                                int lanes, hipStream_t st)
 {
     const long n = p.s_end - p.s_begin;
     if (n <= 0 || p.npix_beam <= 0)
@@ -2244,23 +2431,30 @@ hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nb
     B.chunk = (int)std::max<long>(std::min<long>(chunk, std::max<long>(fill, 64)), 1);
     B.nchunks = (n + B.chunk - 1) / B.chunk;
     const size_t bytes = sizeof(BandRunTable) * kAdjWaves + 6 * sizeof(float) * (size_t)B.rows;
+    if (lanes != 64 && lanes != 8)
+        return hipErrorInvalidValue;
+    const void *fn = lanes == 8 ? (const void *)k_adjbeam_band_sub<8> : (const void *)k_adjbeam_band;
     constexpr int kDevs = 64;
-    static std::atomic<bool> configured[kDevs];
+    static std::atomic<bool> configured[2][kDevs];
+    const int which = lanes == 8 ? 1 : 0;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess)
         return e;
-    if (dev < 0 || dev >= kDevs || !configured[dev].load()) {
-        e = hipFuncSetAttribute((const void *)k_adjbeam_band,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (dev < 0 || dev >= kDevs || !configured[which][dev].load()) {
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess)
             return e;
         if (dev >= 0 && dev < kDevs)
-            configured[dev].store(true);
+            configured[which][dev].store(true);
     }
     const long items = B.nchunks * B.nband;
     const long g = nblocks > 0 ? std::min<long>(items, nblocks) : items;   /* 0: one per item */
-    hipLaunchKernelGGL(k_adjbeam_band, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, B);
+    if (lanes == 8)
+        hipLaunchKernelGGL(k_adjbeam_band_sub<8>, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p,
+                           B);
+    else
+        hipLaunchKernelGGL(k_adjbeam_band, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, B);
     return hipGetLastError();
 }
 

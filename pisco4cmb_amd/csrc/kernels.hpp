@@ -144,7 +144,7 @@ hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st);
  * samples); p.gbeam accumulates in FP64, p.sky4 holds the packed sky.  rows:
  * rows per band (0 or more than fits: as many as fit the LDS). */
 hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks, int ncu,
-                                hipStream_t st);
+                                int lanes, hipStream_t st);
 int adjbeam_band_rows();
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */

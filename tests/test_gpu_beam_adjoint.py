@@ -1,10 +1,10 @@
 """The beam adjoint (TOD -> beam maps) in LDS bands of beam rows (tuning key
-adjbeam_bands = 1, the default) against the oracle's FP64 beam adjoint and
-against the global-atomic form (adjbeam_bands = 0), relative RMS 1e-6 per
-column: the headline beam shape (81 bands), a cap reaching the equatorial
-rings of a coarse beam map (bands of 300 rows: whole and partial rings),
-small and large work items, samples near the poles; and the dot product
-with the forward.""" 
+adjbeam_bands = 1, the default; 8 or 64 lanes per sample, adjbeam_lanes)
+against the oracle's FP64 beam adjoint and against the global-atomic form
+(adjbeam_bands = 0), relative RMS 1e-6 per column: the headline beam shape
+(81 bands), a cap reaching the equatorial rings of a coarse beam map (bands
+of 300 rows: whole and partial rings), small and large work items, samples
+near the poles; and the dot product with the forward."""
 import math
 
 import numpy as np
@@ -39,15 +39,17 @@ def check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, chunks=(1024,), row
     conv = conv_for(ns, sky, beam, ra, dec, pa)
     conv.set_tuning("adjbeam_rows", rows)
     res = {}
-    for bands, chunk in [(1, c) for c in chunks] + [(0, 1024)]:
+    for bands, chunk, lanes in ([(1, c, 8) for c in chunks] + [(1, chunks[0], 64)] +
+                                [(0, 1024, 8)]):
         conv.set_tuning("adjbeam_bands", bands)
         conv.set_tuning("adjbeam_chunk", chunk)
+        conv.set_tuning("adjbeam_lanes", lanes)
         oa, ob = conv.exec_adjoint_beam(ga, gb)
         for k, (got, col) in enumerate(((oa[0], 0), (oa[1], 1), (oa[2], 2),
                                         (ob[0], 3), (ob[1], 4), (ob[2], 5))):
-            assert rel_rms(got, G[:, col]) < 1e-6, (bands, chunk, k)
+            assert rel_rms(got, G[:, col]) < 1e-6, (bands, chunk, lanes, k)
         assert np.all(oa[3] == 0) and np.all(ob[3] == 0)
-        res[(bands, chunk)] = (oa, ob)
+        res[(bands, chunk, lanes)] = (oa, ob)
     return conv, ga, gb, res
 
 
