@@ -98,3 +98,27 @@ def test_dot_product(oracle):
     rhs = float(np.sum(np.stack(oa).astype(np.float64) * np.stack(beam.aBeams)) +
                 np.sum(np.stack(ob).astype(np.float64) * np.stack(beam.bBeams)))
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+@pytest.mark.parametrize("npix,rows", [(4, 0), (60, 0), (301, 300)])
+def test_small_beams_and_one_cotangent(oracle, npix, rows):
+    """A beam of one ring (4 rows), of a few rings, and a last band of one
+    row (301 rows in bands of 300), with g_b absent: against the oracle."""
+    ns = 16
+    sky = inputs.iid_sky(ns, seed=87)
+    b = pypisco.PolBeam(64, npix, 0.0, "a")
+    b.make_unpol_gaussian_elliptical_beams(3.0, 2.5, 0.0)
+    b.build(ns)
+    ra, dec, pa = inputs.raster_scan(500, size_deg=30.0)
+    rng = np.random.default_rng(88)
+    ga = rng.normal(size=ra.size).astype(np.float32)
+    beams = list(b.aBeams) + list(b.bBeams)
+    G = oracle.adjoint_beam(ns, *sky[:3], 64, beams, ra, dec, pa, ga, None)
+    conv = conv_for(ns, sky, b, ra, dec, pa)
+    conv.set_tuning("adjbeam_rows", rows)
+    for lanes in (8, 64):
+        conv.set_tuning("adjbeam_lanes", lanes)
+        oa, ob = conv.exec_adjoint_beam(ga, None)
+        for k in range(3):
+            assert rel_rms(oa[k], G[:, k]) < 1e-6
+        assert np.all(ob[0] == 0) and np.all(ob[1] == 0) and np.all(ob[2] == 0)
