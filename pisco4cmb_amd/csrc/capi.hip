@@ -94,6 +94,7 @@ struct pisco_ctx {
     pisco_run_config cfg{};
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;   /* host-API TOD copies, overlapped with the forward */
 
     bool has_sky = false, has_beam = false, has_scan = false;
     int nside_sky = 0;
@@ -239,7 +240,8 @@ int adj_lanes(const pisco_ctx *c, const PairParams &p)
     return disc_pixels(p) < 1000.0 ? 8 : 64;
 }
 
-/* Launches op over all samples in chunks of ptgPerConv (or one launch). */
+/* Launches op over samples p.s_begin .. p.s_end in chunks of ptgPerConv (or
+ * one launch). */
 int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st, bool sky_packed = false)
 {
     if (op == OP_FWD && !c->beam_interp && sky_packed) {
@@ -253,11 +255,11 @@ int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st, bool sky_packed
         p.sky4 = (const float4 *)c->sky4.p;
     }
     const int lanes = (op == OP_FWD && !c->beam_interp) ? fwd_lanes(c, p) : 64;
-    long n = p.s_end;
-    long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : n;
+    const long n = p.s_end, s0 = p.s_begin;
+    long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : n - s0;
     if (chunk < (1L << 20))
-        chunk = std::max<long>(chunk, std::min<long>(n, 1L << 20));
-    for (long s = 0; s < n; s += chunk) {
+        chunk = std::max<long>(chunk, std::min<long>(n - s0, 1L << 20));
+    for (long s = s0; s < n; s += chunk) {
         p.s_begin = s;
         p.s_end = std::min(n, s + chunk);
         if (c->beam_interp)
@@ -658,6 +660,10 @@ void pisco_destroy(pisco_ctx *ctx)
     ctx->fxscale.release();
     for (auto &kv : ctx->ring_tabs)
         (void)hipFree(kv.second);
+    if (ctx->copy_stream) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamDestroy(ctx->copy_stream);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -766,12 +772,54 @@ int pisco_exec_convolution(pisco_ctx *c, char polflag, float *tod_a, float *tod_
                    (const float *)c->beam.p, n, c->ra.p, c->dec.p, c->pa.p, p));
     p.tod_a = c->tod.p;
     p.tod_b = c->tod.p + n;
-    CHK(run_pair(c, OP_FWD, p, c->stream));
-    if (polflag != 'b')
-        HIPCHK(hipMemcpyAsync(tod_a, c->tod.p, n * 4, hipMemcpyDeviceToHost, c->stream));
-    if (polflag != 'a')
-        HIPCHK(hipMemcpyAsync(tod_b, c->tod.p + n, n * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    /* The forward in up to 4 launches, each followed by an event; the copies
+     * of a launch's TOD to the host run on a second stream behind its event,
+     * so they overlap the launches after it.  A sample's TOD does not depend
+     * on how the samples are split into launches. */
+    if (!c->copy_stream)
+        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    const int K = n >= (1L << 18) ? 4 : 1;
+    const long part = (n + K - 1) / K;
+    hipEvent_t ev[4];
+    int nev = 0;
+    int rc = PISCO_OK;
+    for (int k = 0; k < K && rc == PISCO_OK; k++) {
+        PairParams q = p;
+        q.s_begin = k * part;
+        q.s_end = std::min(n, (k + 1) * part);
+        if (q.s_begin >= q.s_end)
+            break;
+        rc = run_pair(c, OP_FWD, q, c->stream, k > 0);
+        if (rc == PISCO_OK) {
+            hipError_t e = hipEventCreateWithFlags(&ev[nev], hipEventDisableTiming);
+            if (e == hipSuccess)
+                e = hipEventRecord(ev[nev], c->stream);
+            if (e != hipSuccess)
+                rc = fail(PISCO_ERR_DEVICE, "event failed: %s", hipGetErrorString(e));
+            else
+                nev++;
+        }
+    }
+    for (int k = 0; k < nev && rc == PISCO_OK; k++) {
+        const long b = k * part, e_ = std::min(n, (k + 1) * part);
+        hipError_t e = hipStreamWaitEvent(c->copy_stream, ev[k], 0);
+        if (e == hipSuccess && polflag != 'b')
+            e = hipMemcpyAsync(tod_a + b, c->tod.p + b, (e_ - b) * 4, hipMemcpyDeviceToHost,
+                               c->copy_stream);
+        if (e == hipSuccess && polflag != 'a')
+            e = hipMemcpyAsync(tod_b + b, c->tod.p + n + b, (e_ - b) * 4, hipMemcpyDeviceToHost,
+                               c->copy_stream);
+        if (e != hipSuccess)
+            rc = fail(PISCO_ERR_DEVICE, "TOD copy failed: %s", hipGetErrorString(e));
+    }
+    hipError_t es = hipStreamSynchronize(c->copy_stream);
+    hipError_t ek = hipStreamSynchronize(c->stream);
+    for (int k = 0; k < nev; k++)
+        (void)hipEventDestroy(ev[k]);
+    if (rc != PISCO_OK)
+        return rc;
+    HIPCHK(es);
+    HIPCHK(ek);
     return PISCO_OK;
 }
 
