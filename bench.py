@@ -219,8 +219,8 @@ def main():
         b_adjb = v_pre * (12 + 24 * k) + nbp * (12 + 4 * k)
         beam_adj = {"samples": nbp, "s": t_adjb, "tod_samples_per_s": k * nbp / t_adjb,
                     "hbm_frac": b_adjb / t_adjb / HBM_PEAK,
-                    "note": "LDS bands of beam rows, 8 samples per wave; the per-band "
-                            "annulus set-up dominates"}
+                    "note": "LDS bands of beam rows, 8 samples per wave; the visit walks "
+                            "of the band annuli dominate"}
         del gbeam8
 
     dominant = "adjoint_map" if t_adj >= t_fwd else "forward"

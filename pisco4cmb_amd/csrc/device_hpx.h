@@ -404,23 +404,78 @@ __host__ __device__ inline int disc_ring_span(const Hpx &h, const Disc &d, int r
     return 1;
 }
 
+/* disc_ring_span with the arc's half-width dphi from a float atan2, for
+ * callers that need a superset (grow) or a subset (!grow) of the exact span
+ * rather than the span itself.  The float dphi is within 4e-6 rad of the
+ * FP64 one (float rounding of x and sqrt(ysq), <= 1.2e-7 rad; plf_atan2_core
+ * with the hardware reciprocal, ~1e-6), which moves the floor arguments a, b
+ * by at most m = nr/(2 pi) 4e-6 pixels.  A floor whose argument lies farther
+ * than m from an integer equals the exact one; otherwise it is moved one
+ * pixel outward (grow) or inward.  Everything else (the ring range, ysq and
+ * the empty-arc test) is the exact path's.  Rings of more than 2^17 pixels
+ * (m > 0.08) take the exact path. */
+__device__ inline int disc_ring_span_cover(const Hpx &h, const Disc &d, int r, bool grow,
+                                           int &ip_lo, int &ip_hi, int &ipix1, int &nr)
+{
+    bool shifted;
+    ring_info(h, r, ipix1, nr, shifted);
+    if (nr > (1 << 17))
+        return disc_ring_span(h, d, r, ip_lo, ip_hi, ipix1, nr);
+    if (r < d.irmin || r > d.irmax)
+        return 2;
+    double z = ring2z(h, r);
+    double x = (d.cosr - z * d.z0) * d.xa;
+    double ysq = 1 - z * z - x * x;
+    if (!(ysq > 0))
+        return 0;   /* dphi = 0 on the exact path: empty */
+    const float yf = (float)sqrt(ysq), xf = (float)x;
+    const float dphi = plf_atan2_core(yf, xf, __builtin_amdgcn_rcpf(fmaxf(fabsf(xf), yf)));
+    const double k = nr * kInvTwoPi, shift = shifted ? 0.5 : 0.;
+    const double a = k * (d.phi - (double)dphi) - shift;
+    const double b = k * (d.phi + (double)dphi) - shift;
+    const double fa = floor(a), fb = floor(b), m = k * 4e-6;
+    const double da = a - fa, db = b - fb;
+    ip_lo = (int)fa + 1;
+    ip_hi = (int)fb;
+    if (da < m || da > 1.0 - m)
+        ip_lo += grow ? -1 : 1;
+    if (db < m || db > 1.0 - m)
+        ip_hi += grow ? 1 : -1;
+    if (ip_lo > ip_hi)
+        return 0;
+    if (ip_hi - ip_lo + 1 >= nr)
+        return 2;
+    return 1;
+}
+
 /* Pixel runs of ring r whose centres lie in the annulus dout minus din (two
  * discs about the same centre, din the smaller; inner == false: no inner
  * disc): up to two arcs, each split at the ring's first pixel, so up to 4
  * runs [p0[k], p0[k] + len[k]).  Returns the number of runs.  Both spans come
  * from disc_ring_span about the same centre, so the inner span lies inside
- * the outer one in the same unwrapped positions. */
+ * the outer one in the same unwrapped positions.  COVER: the spans come from
+ * disc_ring_span_cover, the outer one grown and the inner one shrunk, so the
+ * runs hold every pixel of the exact annulus (and, near a tie of a floor,
+ * a pixel more); the nesting of the spans is kept. */
+template <bool COVER = false>
 __host__ __device__ inline int annulus_ring(const Hpx &h, const Disc &dout, const Disc &din,
                                             bool inner, int r, int p0[4], int len[4])
 {
     int lo_o = 0, hi_o = -1, ip1 = 0, nr = 0;
-    const int kind_o = disc_ring_span(h, dout, r, lo_o, hi_o, ip1, nr);
+    int kind_o;
+    if constexpr (COVER)
+        kind_o = disc_ring_span_cover(h, dout, r, true, lo_o, hi_o, ip1, nr);
+    else
+        kind_o = disc_ring_span(h, dout, r, lo_o, hi_o, ip1, nr);
     if (kind_o == 0)
         return 0;
     int kind_i = 0, lo_i = 0, hi_i = -1;
     if (inner && r >= din.r_first && r <= din.r_last) {
         int ip1b, nrb;
-        kind_i = disc_ring_span(h, din, r, lo_i, hi_i, ip1b, nrb);
+        if constexpr (COVER)
+            kind_i = disc_ring_span_cover(h, din, r, false, lo_i, hi_i, ip1b, nrb);
+        else
+            kind_i = disc_ring_span(h, din, r, lo_i, hi_i, ip1b, nrb);
     }
     if (kind_i == 2)
         return 0;   /* the whole ring lies inside the inner disc */

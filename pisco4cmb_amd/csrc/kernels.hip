@@ -1621,7 +1621,7 @@ __device__ inline int build_band_runs(const Hpx &hs, const Disc &dout, const Dis
     int p0[4] = {0, 0, 0, 0}, ln[4] = {0, 0, 0, 0};
     int ns = 0;
     if (r <= dout.r_last)
-        ns = annulus_ring(hs, dout, din, inner, r, p0, ln);
+        ns = annulus_ring<true>(hs, dout, din, inner, r, p0, ln);
     int tot = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++)
@@ -1710,7 +1710,7 @@ __device__ inline int build_band_sub(const Hpx &hs, const Disc &dout, const Disc
     int p0[4] = {0, 0, 0, 0}, ln[4] = {0, 0, 0, 0};
     int ns = 0;
     if (ps * SUB + sl < nrings)
-        ns = annulus_ring(hs, dout, din, inner, r, p0, ln);
+        ns = annulus_ring<true>(hs, dout, din, inner, r, p0, ln);
     int tot = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++)
