@@ -414,8 +414,8 @@ __host__ __device__ inline int disc_ring_span(const Hpx &h, const Disc &d, int r
  * pixel outward (grow) or inward.  Everything else (the ring range, ysq and
  * the empty-arc test) is the exact path's.  Rings of more than 2^17 pixels
  * (m > 0.08) take the exact path. */
-__device__ inline int disc_ring_span_cover(const Hpx &h, const Disc &d, int r, bool grow,
-                                           int &ip_lo, int &ip_hi, int &ipix1, int &nr)
+__host__ __device__ inline int disc_ring_span_cover(const Hpx &h, const Disc &d, int r, bool grow,
+                                                    int &ip_lo, int &ip_hi, int &ipix1, int &nr)
 {
     bool shifted;
     ring_info(h, r, ipix1, nr, shifted);
@@ -429,7 +429,12 @@ __device__ inline int disc_ring_span_cover(const Hpx &h, const Disc &d, int r, b
     if (!(ysq > 0))
         return 0;   /* dphi = 0 on the exact path: empty */
     const float yf = (float)sqrt(ysq), xf = (float)x;
-    const float dphi = plf_atan2_core(yf, xf, __builtin_amdgcn_rcpf(fmaxf(fabsf(xf), yf)));
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float rm = __builtin_amdgcn_rcpf(fmaxf(fabsf(xf), yf));
+#else
+    const float rm = 1.0f / fmaxf(fabsf(xf), yf);   /* host builds (checks of the cover) */
+#endif
+    const float dphi = plf_atan2_core(yf, xf, rm);
     const double k = nr * kInvTwoPi, shift = shifted ? 0.5 : 0.;
     const double a = k * (d.phi - (double)dphi) - shift;
     const double b = k * (d.phi + (double)dphi) - shift;
