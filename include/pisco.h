@@ -385,6 +385,18 @@ long pisco_ang2pix(int nside, double theta, double phi);
 /* Healpix_Base::pix2ang on the host.  Returns PISCO_OK or an error. */
 int pisco_pix2ang(int nside, long pix, double *theta, double *phi);
 
+/* The sky pixels (NSIDE nside_sky, RING) that the banded beam adjoint walks
+ * for band `band` (beam rows [band*rows, min(npix_beam, (band+1)*rows)) of
+ * a beam map of NSIDE nside_beam and npix_beam pixels) around the pointing
+ * (ra, dec): the annulus runs of every sky ring, as the kernels form them
+ * (cover != 0: the float-span form the kernels use; 0: exact spans).  Writes
+ * at most cap pixel indices to pix, in walk order, and returns their
+ * number; -1 on bad arguments, -2 if more than cap.  Host only.  No
+ * counterpart in the reference (its beam maps have no gradient); it exposes
+ * the decomposition so that its coverage can be checked. */
+long pisco_band_annulus(int nside_sky, int nside_beam, long npix_beam, int rows, int band,
+                        float ra, float dec, int cover, long cap, int *pix);
+
 /* E-field magnitude and phase of a perfectly co-polarised elliptical
  * Gaussian beam on the first npix pixels of a beam map
  * (PolBeam::make_unpol_gaussian_elliptical_beams, polbeam.cpp:290-372;

@@ -26,9 +26,9 @@ PISCO_ERR_STATE = 2
 PISCO_ERR_MEMORY = 3
 PISCO_ERR_DEVICE = 4
 
-c_int, c_long, c_double, c_char, c_void_p, c_size_t = (
-    ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_char, ctypes.c_void_p,
-    ctypes.c_size_t)
+c_int, c_long, c_double, c_float, c_char, c_void_p, c_size_t = (
+    ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_float, ctypes.c_char,
+    ctypes.c_void_p, ctypes.c_size_t)
 P = c_void_p  # all array arguments are passed as raw addresses
 
 
@@ -108,6 +108,8 @@ SIGNATURES = {
     "pisco_beam_rho_max": (c_double, [c_int, c_long]),
     "pisco_ang2pix": (c_long, [c_int, c_double, c_double]),
     "pisco_pix2ang": (c_int, [c_int, c_long, P, P]),
+    "pisco_band_annulus": (c_long, [c_int, c_int, c_long, c_int, c_int, c_float, c_float, c_int,
+                                    c_long, P]),
     "pisco_beam_gaussian_fields": (c_int, [c_int, c_long, c_double, c_double, c_double, P,
                                            P, P, P]),
     "pisco_beam_from_fields": (c_int, [c_long, P, P, P, P, P, P, P, P]),

@@ -364,6 +364,43 @@ double pisco_beam_rho_max(int nside_beam, long npix)
     return th;
 }
 
+long pisco_band_annulus(int nside_sky, int nside_beam, long npix_beam, int rows, int band,
+                        float ra, float dec, int cover, long cap, int *pix)
+{
+    if (!valid_nside(nside_sky) || !valid_nside(nside_beam) || npix_beam < 1 ||
+        npix_beam >= 12L * nside_beam * nside_beam || rows < 1 || band < 0 ||
+        (long)band * rows >= npix_beam || cap < 0 || (cap > 0 && !pix))
+        return -1;
+    const Hpx hs = hpx_make(nside_sky), hb = hpx_make(nside_beam);
+    const double rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
+    const int lo = band * rows;
+    const int nrow = (int)std::min<long>(npix_beam - lo, rows);
+    bool inner;
+    double r_in, c_in, r_out, c_out;
+    band_annulus_radii(hb, rho_max, cos_radius(rho_max), lo, nrow, inner, r_in, c_in, r_out,
+                       c_out);
+    const Centre c = centre_make(ra, dec, 0.0f);
+    Disc dout, din;
+    disc_setup(hs, kHalfPi - c.dec, c.ra, r_out, c_out, dout);
+    if (inner)
+        disc_setup(hs, kHalfPi - c.dec, c.ra, r_in, c_in, din);
+    else
+        din = dout;
+    long n = 0;
+    for (int r = dout.r_first; r <= dout.r_last; r++) {
+        int p0[4], len[4];
+        const int k = cover ? annulus_ring<true>(hs, dout, din, inner, r, p0, len)
+                            : annulus_ring<false>(hs, dout, din, inner, r, p0, len);
+        for (int q = 0; q < k; q++)
+            for (int v = 0; v < len[q]; v++) {
+                if (n >= cap)
+                    return -2;
+                pix[n++] = p0[q] + v;
+            }
+    }
+    return n;
+}
+
 long pisco_ang2pix(int nside, double theta, double phi)
 {
     if (!valid_nside(nside) || !(theta >= 0.0 && theta <= kPi) || !(fabs(phi) < 1e300))

@@ -529,6 +529,29 @@ __host__ __device__ inline int annulus_ring(const Hpx &h, const Disc &dout, cons
     return n;
 }
 
+/* The annulus of sky radii that holds every visit whose beam pixel lies in
+ * rows [lo, lo + nrow) of a beam map hb of support radius rho_max (cos_rmax
+ * its cosine): a point in a pixel of beam ring R lies between the centres of
+ * rings R-1 and R+1, so rows of rings Rf..Rl need rho in
+ * [theta(Rf-1), theta(Rl+1)], widened by 1e-9 and clipped to rho_max.
+ * inner == false: no inner disc (Rf is ring 1). */
+__host__ __device__ inline void band_annulus_radii(const Hpx &hb, double rho_max, double cos_rmax,
+                                                   int lo, int nrow, bool &inner, double &r_in,
+                                                   double &c_in, double &r_out, double &c_out)
+{
+    const int rf = pix2ring(hb, lo), rl = pix2ring(hb, lo + nrow - 1);
+    inner = rf > 1;
+    r_in = inner ? ring_theta(hb, rf - 1) - 1e-9 : 0.0;
+    r_out = rl + 1 <= 4 * hb.nside - 1 ? ring_theta(hb, rl + 1) + 1e-9 : rho_max;
+    if (r_out >= rho_max) {
+        r_out = rho_max;
+        c_out = cos_rmax;
+    } else {
+        c_out = pl_cos(r_out);
+    }
+    c_in = pl_cos(r_in);
+}
+
 /* ---- the beam-frame mapping of one visit -------------------------------- */
 
 /* Per-sample constants of the rotation (sphtrigo.cpp:29-30 evaluated once

@@ -1783,23 +1783,6 @@ __device__ inline void walk_band_sub(const PairParams &p, const SubBandTable<SUB
         use(q);
 }
 
-/* Band of rows [lo, lo + nrow) of the beam map: the annulus radii. */
-__device__ inline void band_radii(const PairParams &p, int lo, int nrow, bool &inner, double &r_in,
-                                  double &c_in, double &r_out, double &c_out)
-{
-    const int rf = pix2ring(p.hb, lo), rl = pix2ring(p.hb, lo + nrow - 1);
-    inner = rf > 1;
-    r_in = inner ? ring_theta(p.hb, rf - 1) - 1e-9 : 0.0;
-    r_out = rl + 1 <= 4 * p.hb.nside - 1 ? ring_theta(p.hb, rl + 1) + 1e-9 : p.rho_max;
-    if (r_out >= p.rho_max) {
-        r_out = p.rho_max;
-        c_out = p.cos_rmax;
-    } else {
-        c_out = pl_cos(r_out);
-    }
-    c_in = pl_cos(r_in);
-}
-
 template <int SUB>
 __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band_sub(PairParams p, BeamBands B)
 {
@@ -1822,7 +1805,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band_sub(PairParams p, Be
         const int nrow = min(p.npix_beam - lo, B.rows);
         bool inner;
         double r_in, c_in, r_out, c_out;
-        band_radii(p, lo, nrow, inner, r_in, c_in, r_out, c_out);
+        band_annulus_radii(p.hb, p.rho_max, p.cos_rmax, lo, nrow, inner, r_in, c_in, r_out, c_out);
         for (int i = tid; i < 6 * nrow; i += kAdjBlock)
             acc[i] = 0.f;
         __syncthreads();
@@ -1895,18 +1878,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band(PairParams p, BeamBa
         const int lo = k * B.rows;
         const int nrow = min(p.npix_beam - lo, B.rows);
         /* the annulus of rows [lo, lo + nrow) */
-        const int rf = pix2ring(p.hb, lo), rl = pix2ring(p.hb, lo + nrow - 1);
-        const bool inner = rf > 1;
-        const double r_in = inner ? ring_theta(p.hb, rf - 1) - 1e-9 : 0.0;
-        double r_out = rl + 1 <= 4 * p.hb.nside - 1 ? ring_theta(p.hb, rl + 1) + 1e-9 : p.rho_max;
-        double c_out;
-        if (r_out >= p.rho_max) {
-            r_out = p.rho_max;
-            c_out = p.cos_rmax;
-        } else {
-            c_out = pl_cos(r_out);
-        }
-        const double c_in = pl_cos(r_in);
+        bool inner;
+        double r_in, c_in, r_out, c_out;
+        band_annulus_radii(p.hb, p.rho_max, p.cos_rmax, lo, nrow, inner, r_in, c_in, r_out,
+                           c_out);
         for (int i = tid; i < 6 * nrow; i += kAdjBlock)
             acc[i] = 0.f;
         __syncthreads();
