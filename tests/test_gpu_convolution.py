@@ -164,3 +164,38 @@ def test_host_api_split_copies(oracle, polflag):
     k = slice(n - 500, n)
     oa, ob = oracle.convolve(ns, *sky[:3], nb, beams_of(beam), ra[k], dec[k], pa[k])
     assert rel_rms(da[k], oa) < 1e-6 and rel_rms(db[k], ob) < 1e-6
+
+
+def test_host_api_split_copies_interpolation(oracle):
+    """The same 4-launch host-API forward in beam interpolation mode: 2^18 + 7
+    samples, whose last 3,000 equal a host-API call over those samples alone
+    (one launch) bit for bit, and whose last 300 match the oracle's
+    interpolated forward."""
+    ns, nb, n = 64, 512, (1 << 18) + 7
+    sky = inputs.iid_sky(ns, seed=73)
+    beam = inputs.gaussian_polbeam(nb, 4.0, 1.5, 0.08, 0.02, nside_sky=ns)
+    ra, dec, pa = inputs.ces_scan(n, t0=900.0)
+
+    def run(r, d, q):
+        conv = pypisco.GPUConvolver(pypisco.default_run_config(ptgPerConv=r.size))
+        conv.configure_execution()
+        conv.set_tuning("beam_interpolation", 1)
+        conv.update_sky(pypisco.Sky(ns, *sky[:3]))
+        conv.update_beam(beam)
+        conv.update_scan(pypisco.Scan(r.size, r, d, q))
+        a = np.zeros(r.size, np.float32)
+        b = np.zeros(r.size, np.float32)
+        conv.exec_convolution(a, b)
+        return a, b
+
+    fa, fb = run(ra, dec, pa)
+    k = slice(n - 3000, n)
+    ta, tb = run(ra[k], dec[k], pa[k])
+    assert np.array_equal(fa[k], ta) and np.array_equal(fb[k], tb)
+    m = slice(n - 300, n)
+    oracle.set_beam_interpolation(True)
+    try:
+        oa, ob = oracle.convolve(ns, *sky[:3], nb, beams_of(beam), ra[m], dec[m], pa[m])
+    finally:
+        oracle.set_beam_interpolation(False)
+    assert rel_rms(fa[m], oa) < 1e-6 and rel_rms(fb[m], ob) < 1e-6
