@@ -613,35 +613,106 @@ def map_maker_workload(dev, nside_sky, n=250_000, ndet=64, reps=3, with_cpu=True
     return res
 
 
+def cpu_threads():
+    """(threads, report): the CPUs this process may run on (its affinity
+    mask, the host cores the job is given), next to nproc and the OpenMP
+    limit of the environment, which the reference's OpenMP loop
+    (convolver.cpp:38-41) would also obey."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:   # pragma: no cover - non-Linux
+        allowed = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = allowed
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    quota = None
+    try:   # cgroup v2 CPU quota, in CPUs
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    return threads, {"nproc": nproc, "affinity_cpus": allowed, "omp_num_threads": omp or None,
+                     "cgroup_cpu_quota": quota}
+
+
+def median_time(fn, reps=3):
+    """One warm-up call, then the median of reps timed calls (seconds)."""
+    fn(warm=True)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn(warm=False)
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2], ts
+
+
 def cpu_baseline(args, I, Q, U, nside_sky, nside_beam, beam, ra, dec, pa, g, tod_gpu=None):
-    """The CPU oracle (a C restatement of the reference Convolver, OpenMP over
-    samples) on a prefix of the same workload: forward + adjoint-map.  With
-    the GPU TOD of the timed steps (tod_gpu = (a, b)), also the parity of
-    that TOD on the prefix: relative RMS against the CPU Convolver and
-    against the all-FP64 oracle, next to the CPU Convolver's own error
-    against the FP64 oracle."""
+    """The CPU oracle (a C restatement of the reference Convolver,
+    convolver.cpp:22-183, OpenMP over samples) on every CPU this process may
+    use, one warm-up and the median of 3 runs:
+      - value: forward + adjoint-map on the first cpu_samples samples of the
+        headline workload (the metric's workload);
+      - c1_full: configuration 1 in full (sky NSIDE 128, beam NSIDE 1024,
+        cap 2.5 deg, 1e4 raster samples), forward;
+      - c2_prefix: the first 1e5 samples of configuration 2 (sky NSIDE 256,
+        beam NSIDE 2048, cap 1.5 deg, CES scan), forward.
+    With the GPU TOD of the timed steps (tod_gpu = (a, b)), also the parity of
+    that TOD on a prefix: relative RMS against the CPU Convolver and against
+    the all-FP64 oracle, next to the CPU Convolver's own error against the
+    FP64 oracle."""
+    import numpy as np
+
+    from pisco4cmb_amd import inputs
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind
     orc = oracle_bind.Oracle()
+    threads, hosts = cpu_threads()
     m = min(args.cpu_samples, ra.size)
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
     beams = list(beam.aBeams) + list(beam.bBeams)
-    t0 = time.time()
-    orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:m], dec[:m], pa[:m], nthreads=cores)
-    t1 = time.time()
-    orc.adjoint_map(nside_sky, nside_beam, beams, ra[:m], dec[:m], pa[:m], g[0, :m], g[1, :m])
-    t2 = time.time()
-    out = {"value": 2 * m / (t2 - t0), "unit": "TOD samples/s", "cores": cores,
+
+    def c3(warm):
+        k = min(m, 2000) if warm else m
+        orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:k], dec[:k], pa[:k],
+                     nthreads=threads)
+        orc.adjoint_map(nside_sky, nside_beam, beams, ra[:k], dec[:k], pa[:k], g[0, :k],
+                        g[1, :k])
+    t3, ts3 = median_time(c3)
+    out = {"value": 2 * m / t3, "unit": "TOD samples/s", "cores": threads,
            "kind": "port",
-           "sample": "first %d samples of the same workload, forward (%.1f s) + "
-                     "adjoint-map (%.1f s), OpenMP over samples" % (m, t1 - t0, t2 - t1)}
+           "sample": "first %d samples of the same workload, forward + adjoint-map, OpenMP "
+                     "over samples on %d threads; one warm-up, median of 3 (%s s)"
+                     % (m, threads, ", ".join("%.2f" % x for x in ts3)),
+           "hosts": hosts}
+
+    # configuration 1 in full: CPU plain, 1 detector pair's forward
+    I1, Q1, U1, _ = inputs.iid_sky(128, seed=1)
+    b1 = inputs.gaussian_polbeam(1024, 2.5, 1.0, 0.05, 0.01, nside_sky=128)
+    r1, d1, p1 = inputs.raster_scan(10000)
+    bl1 = list(b1.aBeams) + list(b1.bBeams)
+    t1, ts1 = median_time(lambda warm: orc.convolve(128, I1, Q1, U1, 1024, bl1, r1, d1, p1,
+                                                    nthreads=threads))
+    out["c1_full"] = {"tod_samples_per_s": 2 * r1.size / t1, "s": t1,
+                      "sample": "configuration 1 in full: %d raster samples, forward of the "
+                                "pair" % r1.size, "runs_s": ts1}
+    # configuration 2, first 1e5 samples
+    I2, Q2, U2, _ = inputs.powerlaw_sky(256, seed=77, lmax=256)
+    b2 = inputs.gaussian_polbeam(2048, 1.5, 0.5, 0.0, 0.0, nside_sky=256)
+    r2, d2, p2 = inputs.ces_scan(100_000)
+    bl2 = list(b2.aBeams) + list(b2.bBeams)
+    t2, ts2 = median_time(lambda warm: orc.convolve(256, I2, Q2, U2, 2048, bl2, r2, d2, p2,
+                                                    nthreads=threads))
+    out["c2_prefix"] = {"tod_samples_per_s": 2 * r2.size / t2, "s": t2,
+                        "sample": "first %d samples of configuration 2's CES scan, forward "
+                                  "of the pair" % r2.size, "runs_s": ts2}
     if tod_gpu is not None:
-        import numpy as np
         mp = min(m, 5000)
         ca, cb = orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:mp], dec[:mp], pa[:mp],
-                              nthreads=cores)
+                              nthreads=threads)
         da, db = orc.convolve(nside_sky, I, Q, U, nside_beam, beams, ra[:mp], dec[:mp], pa[:mp],
-                              precision64=True, nthreads=cores)
+                              precision64=True, nthreads=threads)
 
         def rr(x, y):
             x = np.concatenate([np.asarray(v, np.float64) for v in x])

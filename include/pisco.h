@@ -81,6 +81,14 @@ int pisco_create(const pisco_run_config *cfg, pisco_ctx **out);
 /* GPUConvolver::~GPUConvolver (gpuconvolver.cu:172-221). NULL is ignored. */
 void pisco_destroy(pisco_ctx *ctx);
 
+/* No counterpart in the reference (GPUConvolver re-allocates nothing after
+ * its first update, gpuconvolver.cu:279-290,354-367): frozen != 0 pins the
+ * context's device workspace, so that a graph captured from calls on it
+ * keeps valid pointers.  While frozen, a call that would need a larger
+ * workspace fails with PISCO_ERR_STATE instead of re-allocating it;
+ * frozen == 0 lifts the pin. */
+int pisco_freeze_buffers(pisco_ctx *ctx, int frozen);
+
 /* GPUConvolver::update_sky(Sky*) (gpuconvolver.cu:223-269).  V may be NULL. */
 int pisco_update_sky(pisco_ctx *ctx, int nside, const float *I, const float *Q,
                      const float *U, const float *V);

@@ -17,8 +17,7 @@ except Exception:  # pragma: no cover - torch is optional for host-only use
     torch = None
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# PISCO_LIB_PATH selects another build of the same library (A/B timing).
-LIB_PATH = os.environ.get("PISCO_LIB_PATH") or os.path.join(HERE, "libpisco.so")
+LIB_PATH = os.path.join(HERE, "libpisco.so")
 
 PISCO_OK = 0
 PISCO_ERR_INVALID = 1
@@ -51,6 +50,7 @@ SIGNATURES = {
     "pisco_version": (c_int, [ctypes.c_char_p, c_int]),
     "pisco_create": (c_int, [ctypes.POINTER(RunConfig), ctypes.POINTER(c_void_p)]),
     "pisco_destroy": (None, [c_void_p]),
+    "pisco_freeze_buffers": (c_int, [c_void_p, c_int]),
     "pisco_update_sky": (c_int, [c_void_p, c_int, P, P, P, P]),
     "pisco_update_beam": (c_int, [c_void_p, c_int, c_long, P, P]),
     "pisco_update_scan": (c_int, [c_void_p, c_long, P, P, P]),

@@ -52,8 +52,8 @@ def _get(ctx, key):
 def autotune_batch(batch, I, Q, U, ra, dec, pa, nside_sky, sample=65536, reps=2):
     """Picks batch_pair_major, batch_pair_range, batch_group, batch_adj_group
     and adjmap_chunk for a DetectorBatch on this pointing; returns {key:
-    (best, {value: seconds})}."""
-    ctx = _ctx(batch.device)
+    (best, {value: seconds})}.  The keys are set on the batch's own context."""
+    ctx = batch.ctx
     n = min(sample, ra.numel())
     r, d, p = ra[:n], dec[:n], pa[:n]
     g = torch.randn((batch.npairs, 2, n), device=batch.device)
@@ -123,9 +123,11 @@ def autotune_pair(I, Q, U, beam_a, beam_b, ra, dec, pa, nside_sky, nside_beam, s
     return result
 
 
-def tuning(device=None):
-    """The current launch shapes of the device's context."""
-    ctx = _ctx(device or torch.device("cuda", torch.cuda.current_device()))
+def tuning(device=None, batch=None):
+    """The current launch shapes of a DetectorBatch's context, or else of the
+    context that single-pair calls on the device's current stream use."""
+    ctx = batch.ctx if batch is not None else _ctx(
+        device or torch.device("cuda", torch.cuda.current_device()))
     return {k: _get(ctx, k) for k in ("batch_group", "batch_adj_group", "adjmap_chunk",
                                       "adjmap_cap", "adjmap_rw", "adjmap_support",
                                       "forward_lanes", "adjmap_lanes", "batch_pair_major",

@@ -11,6 +11,10 @@ pisco_dev_batched_forward, include/pisco.h).
 beam_a, beam_b: float32 [4, npix] cuda tensors (PolBeam aBeams / bBeams);
 pair k's TOD is what the single-pair forward gives with pa + pa_offset (in
 float).  torch supplies device memory and the current stream only.
+
+Each DetectorBatch owns a library context (its device workspace), shared
+with no other object; BatchedStepGraph freezes that workspace once captured,
+so the graph's kernel arguments stay valid whatever else runs on the device.
 """
 import ctypes
 
@@ -18,7 +22,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib
-from .torch_ops import _ctx, _req, _stream, pack_beam
+from .torch_ops import _req, _stream, new_ctx, pack_beam
 
 
 class DetectorBatch:
@@ -37,7 +41,8 @@ class DetectorBatch:
             table[k].pa_offset = float(pa_off)
         self.npairs = len(pairs)
         self.nside_beams = [int(p[2]) for p in pairs]
-        self.ctx = _ctx(self.device)
+        self.handle = None
+        self.ctx = new_ctx(self.device)
         torch.cuda.synchronize(self.device)   # packed rows complete before the table refers to them
         h = ctypes.c_void_p()
         check(lib.pisco_batch_create(self.ctx, self.npairs, table, ctypes.byref(h)))
@@ -85,10 +90,31 @@ class DetectorBatch:
                  dec.data_ptr(), pa.data_ptr(), g.data_ptr(), *[o.data_ptr() for o in out]))
         return tuple(out)
 
+    def set_tuning(self, key, value):
+        """pisco_set_tuning on the batch's own context (launch shapes of its
+        batched calls)."""
+        check(lib.pisco_set_tuning(self.ctx, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        v = ctypes.c_long()
+        check(lib.pisco_get_tuning(self.ctx, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def freeze(self, frozen=True):
+        """Pin the batch's device workspace (pisco_freeze_buffers): while
+        frozen, a call that needs more workspace raises PiscoError
+        (PISCO_ERR_STATE) instead of re-allocating memory that a captured
+        graph refers to."""
+        check(lib.pisco_freeze_buffers(self.ctx, 1 if frozen else 0))
+
     def close(self):
         if getattr(self, "handle", None):
             lib.pisco_batch_destroy(self.handle)
             self.handle = None
+        if getattr(self, "ctx", None):
+            torch.cuda.synchronize(self.device)
+            lib.pisco_destroy(self.ctx)
+            self.ctx = None
 
     def __del__(self):
         try:
@@ -125,13 +151,17 @@ class BatchedStepGraph:
     hipGraph (torch.cuda.CUDAGraph) and replayed: tod = forward(sky) and
     (gI, gQ, gU) = adjoint(g) on fixed input tensors, whose contents may
     change between replays.  A warm-up call runs first, so no allocation or
-    first-use setup happens during capture."""
+    first-use setup happens during capture; the batch's workspace is then
+    frozen for as long as the graph lives (a larger call on the same batch
+    raises instead of moving memory the graph uses)."""
 
     def __init__(self, batch, I, Q, U, ra, dec, pa, g, nside_sky):
         self.inputs = (I, Q, U, ra, dec, pa, g)
+        self.batch = batch
         batch.forward(I, Q, U, ra, dec, pa, nside_sky)
         batch.adjoint_map(g, ra, dec, pa, nside_sky)
         torch.cuda.synchronize()
+        batch.freeze(True)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self.tod = batch.forward(I, Q, U, ra, dec, pa, nside_sky)

@@ -52,14 +52,23 @@ constexpr int kMaxNside = 8192;   /* int32 pixel indices, as Healpix_Base<int> *
 
 bool valid_nside(int n) { return n >= 1 && n <= kMaxNside; }
 
-/* A grow-only device allocation. */
+/* A grow-only device allocation.  While `frozen` is set (pisco_freeze_buffers:
+ * a captured graph holds p as a kernel argument) a request that would move
+ * the buffer fails instead of freeing it. */
 template <typename T> struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;   /* elements */
+    bool frozen = false;
     int ensure(size_t n)
     {
         if (n <= cap)
             return PISCO_OK;
+        if (frozen)
+            return fail(PISCO_ERR_STATE,
+                        "a %zu-byte scratch buffer must grow to %zu bytes, but the context's "
+                        "buffers are frozen (a captured graph refers to them); use another "
+                        "context for larger calls, or unfreeze after the graph is gone",
+                        cap * sizeof(T), n * sizeof(T));
         if (p)
             (void)hipFree(p);
         p = nullptr;
@@ -88,6 +97,49 @@ int pisco::set_error(int code, const char *msg)
 {
     g_err = msg;
     return code;
+}
+
+int pisco::pair_params(int nside_sky, const float *I, const float *Q, const float *U,
+                       int nside_beam, long npix_beam, const float *beam8, long nsamples,
+                       const float *ra, const float *dec, const float *pa, PairParams &p)
+{
+    if (!valid_nside(nside_sky) || !valid_nside(nside_beam))
+        return fail(PISCO_ERR_INVALID, "NSIDE out of range [1, %d]", kMaxNside);
+    long npb_max = 12L * nside_beam * nside_beam;
+    if (npix_beam < 1 || npix_beam >= npb_max)
+        return fail(PISCO_ERR_INVALID,
+                    "beam npix %ld must be in [1, 12*nside_beam^2 - 1]", npix_beam);
+    if (nsamples < 0)
+        return fail(PISCO_ERR_INVALID, "negative nsamples");
+    memset(&p, 0, sizeof p);
+    p.hs = hpx_make(nside_sky);
+    p.rings = nullptr;
+    p.I = I;
+    p.Q = Q;
+    p.U = U;
+    p.hb = hpx_make(nside_beam);
+    p.npix_beam = (int)npix_beam;
+    p.rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
+    p.cos_rmax = pl_cos(p.rho_max < kPi ? p.rho_max : kPi);
+    p.fc = fast_consts(p.hb);
+    p.fc64 = fast_consts64(p.hb);
+    p.beam = beam8;
+    p.ra = ra;
+    p.dec = dec;
+    p.pa = pa;
+    p.s_begin = 0;
+    p.s_end = nsamples;
+    return PISCO_OK;
+}
+
+/* Measured on MI355X over 1e6 CES samples: 8 lanes took 0.59x the time of 64
+ * at 135 pixels per disc and 0.88x at 540; 16 lanes 0.98x at 1,500; 64 lanes
+ * were fastest at 6,000. */
+int pisco::forward_lanes_auto(const PairParams &p)
+{
+    const double s = pl_sin(0.5 * std::min(p.rho_max, kPi));
+    const double dp = (double)p.hs.npix * s * s;   /* expected pixels per disc */
+    return dp < 1000.0 ? 8 : dp < 3000.0 ? 16 : 64;
 }
 
 struct pisco_ctx {
@@ -133,6 +185,13 @@ struct pisco_ctx {
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
     DevBuf<float> sky4;           /* the forward's packed sky: I, Q, U, 0 per pixel */
+    bool frozen = false;          /* pisco_freeze_buffers */
+
+    template <typename F> void each_buf(F f)
+    {
+        f(I); f(Q); f(U); f(beam); f(ra); f(dec); f(pa); f(tod); f(cot); f(acc); f(fout);
+        f(fxwork); f(fxscale); f(support); f(sky4);
+    }
 };
 
 namespace {
@@ -179,34 +238,11 @@ int setup_pair(pisco_ctx *c, int nside_sky, const float *I, const float *Q,
                long nsamples, const float *ra, const float *dec, const float *pa,
                PairParams &p)
 {
-    if (!valid_nside(nside_sky) || !valid_nside(nside_beam))
-        return fail(PISCO_ERR_INVALID, "NSIDE out of range [1, %d]", kMaxNside);
-    long npb_max = 12L * nside_beam * nside_beam;
-    if (npix_beam < 1 || npix_beam >= npb_max)
-        return fail(PISCO_ERR_INVALID,
-                    "beam npix %ld must be in [1, 12*nside_beam^2 - 1]", npix_beam);
-    if (nsamples < 0)
-        return fail(PISCO_ERR_INVALID, "negative nsamples");
+    CHK(pair_params(nside_sky, I, Q, U, nside_beam, npix_beam, beam8, nsamples, ra, dec, pa,
+                    p));
     const RingRec *rt = nullptr;
     CHK(ring_table(c, nside_sky, &rt));
-    memset(&p, 0, sizeof p);
-    p.hs = hpx_make(nside_sky);
     p.rings = rt;
-    p.I = I;
-    p.Q = Q;
-    p.U = U;
-    p.hb = hpx_make(nside_beam);
-    p.npix_beam = (int)npix_beam;
-    p.rho_max = pisco_beam_rho_max(nside_beam, npix_beam);
-    p.cos_rmax = cos_radius(p.rho_max);
-    p.fc = fast_consts(p.hb);
-    p.fc64 = fast_consts64(p.hb);
-    p.beam = beam8;
-    p.ra = ra;
-    p.dec = dec;
-    p.pa = pa;
-    p.s_begin = 0;
-    p.s_end = nsamples;
     return PISCO_OK;
 }
 
@@ -218,15 +254,12 @@ double disc_pixels(const PairParams &p)
 }
 
 /* Lanes per sample of the nearest-pixel forward: the tuning value, or
- * (auto) from the expected disc size.  Measured on MI355X over 1e6 CES
- * samples: 8 lanes took 0.59x the time of 64 at 135 pixels per disc and
- * 0.88x at 540; 16 lanes 0.98x at 1,500; 64 lanes were fastest at 6,000. */
+ * (auto) forward_lanes_auto. */
 int fwd_lanes(const pisco_ctx *c, const PairParams &p)
 {
     if (c->forward_lanes > 0)
         return c->forward_lanes;
-    const double dp = disc_pixels(p);
-    return dp < 1000.0 ? 8 : dp < 3000.0 ? 16 : 64;
+    return forward_lanes_auto(p);
 }
 
 /* Lanes per sample of the single-pair sky adjoint: the tuning value, or
@@ -680,21 +713,7 @@ void pisco_destroy(pisco_ctx *ctx)
     /* teardown: errors are not reportable here */
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->I.release();
-    ctx->Q.release();
-    ctx->U.release();
-    ctx->beam.release();
-    ctx->ra.release();
-    ctx->dec.release();
-    ctx->pa.release();
-    ctx->tod.release();
-    ctx->cot.release();
-    ctx->acc.release();
-    ctx->fout.release();
-    ctx->support.release();
-    ctx->sky4.release();
-    ctx->fxwork.release();
-    ctx->fxscale.release();
+    ctx->each_buf([](auto &b) { b.release(); });
     for (auto &kv : ctx->ring_tabs)
         (void)hipFree(kv.second);
     if (ctx->copy_stream) {
@@ -703,6 +722,15 @@ void pisco_destroy(pisco_ctx *ctx)
     }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+int pisco_freeze_buffers(pisco_ctx *c, int frozen)
+{
+    if (!c)
+        return fail(PISCO_ERR_INVALID, "NULL context");
+    c->frozen = frozen != 0;
+    c->each_buf([&](auto &b) { b.frozen = c->frozen; });
+    return PISCO_OK;
 }
 
 int pisco_update_sky(pisco_ctx *c, int nside, const float *I, const float *Q,

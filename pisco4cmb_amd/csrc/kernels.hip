@@ -2302,28 +2302,38 @@ size_t adjmap_lds_bytes(int cap, int rw, int group, bool det)
            (det ? 8 + 8 * 3 * (size_t)cap : sizeof(float) * 3 * (size_t)cap);
 }
 
-template <int BG, bool DET, int SUB = kWaveSize>
-static hipError_t launch_adjmap_bg(const PairParams &p, const AdjLayout &L, size_t bytes,
-                                   long g, hipStream_t st)
+/* Raises the dynamic-LDS limit of kernel fn on the current device to at
+ * least `bytes`.  `configured[dev]` holds the largest limit set so far for
+ * that kernel and device, so a later launch that needs more LDS than an
+ * earlier one raises the limit again (it is never lowered). */
+constexpr int kDevs = 64;
+static hipError_t ensure_dyn_lds(const void *fn, size_t bytes, std::atomic<size_t> *configured)
 {
-    /* the dynamic-LDS limit already set for this kernel, per device */
-    constexpr int kDevs = 64;
-    static std::atomic<size_t> configured[kDevs];
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess)
         return e;
-    if (dev < 0 || dev >= kDevs || bytes > configured[dev].load()) {
-        e = hipFuncSetAttribute((const void *)k_adjmap_lds<BG, DET, SUB>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess)
-            return e;
-        if (dev >= 0 && dev < kDevs) {
-            size_t cur = configured[dev].load();
-            while (bytes > cur && !configured[dev].compare_exchange_weak(cur, bytes)) {
-            }
+    if (dev >= 0 && dev < kDevs && bytes <= configured[dev].load())
+        return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess)
+        return e;
+    if (dev >= 0 && dev < kDevs) {
+        size_t cur = configured[dev].load();
+        while (bytes > cur && !configured[dev].compare_exchange_weak(cur, bytes)) {
         }
     }
+    return hipSuccess;
+}
+
+template <int BG, bool DET, int SUB = kWaveSize>
+static hipError_t launch_adjmap_bg(const PairParams &p, const AdjLayout &L, size_t bytes,
+                                   long g, hipStream_t st)
+{
+    static std::atomic<size_t> configured[kDevs];
+    hipError_t e = ensure_dyn_lds((const void *)k_adjmap_lds<BG, DET, SUB>, bytes, configured);
+    if (e != hipSuccess)
+        return e;
     hipLaunchKernelGGL((k_adjmap_lds<BG, DET, SUB>), dim3((unsigned)g), dim3(kAdjBlock), bytes, st,
                        p, L);
     return hipGetLastError();
@@ -2400,6 +2410,7 @@ hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nb
     B.rows = adjbeam_band_rows();
     if (rows > 0 && rows < B.rows)
         B.rows = rows;
+    B.rows = std::min<long>(B.rows, p.npix_beam);   /* no LDS for rows past the beam */
     B.nband = (p.npix_beam + B.rows - 1) / B.rows;
     /* smaller items when there are too few to give every block two */
     const long fill = n * B.nband / (2L * std::max(ncu, 1));
@@ -2409,20 +2420,10 @@ hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nb
     if (lanes != 64 && lanes != 8)
         return hipErrorInvalidValue;
     const void *fn = lanes == 8 ? (const void *)k_adjbeam_band_sub<8> : (const void *)k_adjbeam_band;
-    constexpr int kDevs = 64;
-    static std::atomic<bool> configured[2][kDevs];
-    const int which = lanes == 8 ? 1 : 0;
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
+    static std::atomic<size_t> configured[2][kDevs];
+    hipError_t e = ensure_dyn_lds(fn, bytes, configured[lanes == 8 ? 1 : 0]);
     if (e != hipSuccess)
         return e;
-    if (dev < 0 || dev >= kDevs || !configured[which][dev].load()) {
-        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess)
-            return e;
-        if (dev >= 0 && dev < kDevs)
-            configured[which][dev].store(true);
-    }
     const long items = B.nchunks * B.nband;
     const long g = nblocks > 0 ? std::min<long>(items, nblocks) : items;   /* 0: one per item */
     if (lanes == 8)

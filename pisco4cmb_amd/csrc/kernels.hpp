@@ -206,6 +206,16 @@ hipError_t launch_libm(int op, long n, const double *x, const double *y, double 
  * (host code outside capi.hip). */
 int set_error(int code, const char *msg);
 
+/* A single-pair launch's parameters from the arguments of pisco_dev_forward
+ * and the adjoints, checked (PISCO_ERR_INVALID with the message set
+ * otherwise); p.rings is left NULL for the caller to fill. */
+int pair_params(int nside_sky, const float *I, const float *Q, const float *U, int nside_beam,
+                long npix_beam, const float *beam8, long nsamples, const float *ra,
+                const float *dec, const float *pa, PairParams &p);
+/* Lanes per sample of the nearest-pixel forward from the expected disc size
+ * npix (1 - cos rho_max) / 2: 8 below 1,000 pixels, 16 below 3,000, else 64. */
+int forward_lanes_auto(const PairParams &p);
+
 } /* namespace pisco */
 
 #endif

@@ -11,7 +11,15 @@ Scans (float32 radians, pa counter-clockwise):
   raster_scan        rows x columns over a square patch, pa cycling a list
   ces_scan           constant-elevation scan at a fixed site: azimuth spin,
                      sidereal drift (ra, dec, parallactic angle)
-Beams: gaussian_polbeam -> PolBeam built like polbeam.cpp:290-372 + build().
+Beams:
+  gaussian_polbeam   co-polar elliptical Gaussian pair, built like
+                     polbeam.cpp:290-372 + build() (Q/U leakage only through
+                     epsilon: aBeams[2] = bBeams[2] = 0)
+  crosspolar_polbeam a pair whose E fields carry a cross-polar component with
+                     its own phase (PolBeam::beam_from_fields,
+                     polbeam.cpp:117-184), as GRASP beams do: tilde U, tilde V
+                     and so aBeams[2], bBeams[2], aBeams[3], bBeams[3] are
+                     non-zero, and the a and b fields differ
 
 Everything here is set-up code; the convolution never depends on it.
 """
@@ -211,3 +219,72 @@ def gaussian_polbeam(nside_beam, cap_deg, fwhm_deg, ellipticity=0.0, epsilon=0.0
                                            phi0_deg)
     b.build(nside_sky if nside_sky is not None else nside_beam)
     return b
+
+
+def beam_pixel_angles(nside_beam, npix):
+    """(theta, phi) in float64 of the RING pixels [0, npix) of a map of NSIDE
+    nside_beam (vectorised; for building field models, not for indexing)."""
+    theta_r, start, count, phi0 = _ring_geometry(nside_beam)
+    p = np.arange(int(npix), dtype=np.int64)
+    r = np.searchsorted(start, p, side="right") - 1
+    theta = theta_r[r]
+    phi = phi0[r] + 2.0 * np.pi * (p - start[r]) / count[r]
+    return theta, phi
+
+
+def crosspolar_fields(nside_beam, npix, fwhm_deg, ellipticity=0.0, phi0_deg=0.0, xpol=0.35,
+                      xphase=0.6, twist=0.3, curvature=0.3):
+    """Co- and cross-polar E fields (magnitude, phase) of a feed over the beam
+    pixels, in the form PolBeam::beam_from_fields takes (polbeam.cpp:117-143).
+
+    The co-polar magnitude is the square root of the elliptical Gaussian of
+    make_unpol_gaussian_elliptical_beams (polbeam.cpp:332-357) with a phase
+    that grows as (rho/sigma_x)^2.  The cross-polar magnitude is xpol times
+    the co-polar one, modulated as 0.4 + 0.6 cos^2(sigma - twist), with phase
+    xphase + 0.5 sin(sigma) + curvature/2 (rho/sigma_x)^2.  Returns four
+    float64 arrays magEco, phaseEco, magEcx, phaseEcx."""
+    theta, phi = beam_pixel_angles(nside_beam, npix)
+    sx = fwhm_deg * DEG / 2.35482
+    sy = fwhm_deg * (1.0 - ellipticity) * DEG / 2.35482
+    p0 = phi0_deg * DEG
+    a = math.cos(p0) ** 2 / (2 * sx * sx) + math.sin(p0) ** 2 / (2 * sy * sy)
+    b = -math.sin(2 * p0) / (4 * sx * sx) + math.sin(2 * p0) / (4 * sy * sy)
+    c = math.sin(p0) ** 2 / (2 * sx * sx) + math.cos(p0) ** 2 / (2 * sy * sy)
+    cs, sn = np.cos(phi), np.sin(phi)
+    g = np.exp(-(a * cs * cs + 2.0 * b * cs * sn + c * sn * sn) * theta * theta)
+    u = (theta / sx) ** 2
+    mag_co = np.sqrt(g)
+    phs_co = curvature * u
+    mag_cx = xpol * mag_co * (0.4 + 0.6 * np.cos(phi - twist) ** 2)
+    phs_cx = xphase + 0.5 * sn + 0.5 * curvature * u
+    return mag_co, phs_co, mag_cx, phs_cx
+
+
+def crosspolar_polbeam(nside_beam, cap_deg, fwhm_deg, ellipticity=0.0, epsilon=0.0,
+                       nside_sky=None, xpol=(0.35, 0.25), xphase=(0.6, -0.9), twist=(0.3, 1.2),
+                       phi0_deg=(0.0, 30.0)):
+    """PolBeam pair with cross-polar fields: detector a and b each from
+    crosspolar_fields with their own (xpol, xphase, twist, phi0) through
+    PolBeam.beam_from_fields, then build(); nPixels = ang2pix(nside_beam,
+    cap, 0).  Every beam map of both detectors is non-zero, including
+    aBeams[2], bBeams[2] (tilde U) and aBeams[3], bBeams[3] (tilde V)."""
+    npix = pypisco.beam_npix_for_radius(nside_beam, cap_deg * DEG)
+    bm = pypisco.PolBeam(nside_beam, npix, epsilon, "p")
+    for k, det in enumerate("ab"):
+        f = crosspolar_fields(nside_beam, npix, fwhm_deg, ellipticity, phi0_deg[k], xpol[k],
+                              xphase[k], twist[k])
+        bm.beam_from_fields(det, *f)
+    bm.build(nside_sky if nside_sky is not None else nside_beam)
+    return bm
+
+
+def make_polbeam(kind, nside_beam, cap_deg, fwhm_deg, ellipticity=0.0, epsilon=0.0,
+                 nside_sky=None):
+    """kind "copolar" (gaussian_polbeam) or "crosspolar" (crosspolar_polbeam)."""
+    if kind == "copolar":
+        return gaussian_polbeam(nside_beam, cap_deg, fwhm_deg, ellipticity, epsilon,
+                                nside_sky=nside_sky)
+    if kind == "crosspolar":
+        return crosspolar_polbeam(nside_beam, cap_deg, fwhm_deg, ellipticity, epsilon,
+                                  nside_sky=nside_sky)
+    raise ValueError("beam kind must be 'copolar' or 'crosspolar'")

@@ -24,15 +24,26 @@ from ._lib import check, lib, ptr_array
 _CTX = {}
 
 
-def _ctx(device):
+def new_ctx(device):
+    """A library context (device workspace) of its own on `device`; the
+    caller releases it with lib.pisco_destroy."""
     idx = device.index if device.index is not None else torch.cuda.current_device()
-    if idx not in _CTX:
-        c = ctypes.c_void_p()
-        cfg = _lib.RunConfig(nStreams=1, maxMemUsage=1 << 48, deviceId=idx,
-                             ptgPerConv=1 << 22, pixelsPerDisc=0)
-        check(lib.pisco_create(ctypes.byref(cfg), ctypes.byref(c)))
-        _CTX[idx] = c
-    return _CTX[idx]
+    c = ctypes.c_void_p()
+    cfg = _lib.RunConfig(nStreams=1, maxMemUsage=1 << 48, deviceId=idx,
+                         ptgPerConv=1 << 22, pixelsPerDisc=0)
+    check(lib.pisco_create(ctypes.byref(cfg), ctypes.byref(c)))
+    return c
+
+
+def _ctx(device):
+    """The context of (device, current stream).  A context's workspace is
+    used by one stream at a time, so work that torch issues on two streams
+    gets two contexts and never shares scratch memory."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    key = (idx, torch.cuda.current_stream(idx).cuda_stream)
+    if key not in _CTX:
+        _CTX[key] = new_ctx(torch.device("cuda", idx))
+    return _CTX[key]
 
 
 def _stream(device):

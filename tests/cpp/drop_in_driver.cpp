@@ -12,24 +12,31 @@
  *   aBeams[0..3], bBeams[0..3] [npix_beam];
  *   tod_a, tod_b (GPUConvolver) [nsamples]; tod_a (Convolver, 'a') [nsamples];
  *   int32 flags: 1 if Convolver rejected polFlag 'x' with
- *   std::invalid_argument, 1 if GPUConvolver rejected ptgPerConv = 0 the same way.
+ *   std::invalid_argument, 1 if GPUConvolver rejected ptgPerConv = 0 the same way;
+ *   with "crosspolar": float64 magEco, phsEco, magEcx, phsEcx of detector a,
+ *   then of detector b [npix_beam each].
  *
- * usage: drop_in_driver OUTPUT
+ * usage: drop_in_driver OUTPUT [crosspolar]
+ *   crosspolar: the beams come from PolBeam::beam_from_fields with a
+ *   cross-polar field of its own phase (polbeam.cpp:117-184), so every beam
+ *   map is non-zero; otherwise make_unpol_gaussian_elliptical_beams.
  */
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "pisco.hpp"
 
 int main(int argc, char **argv)
 {
-    if (argc != 2) {
-        std::fprintf(stderr, "usage: %s OUTPUT\n", argv[0]);
+    if (argc != 2 && argc != 3) {
+        std::fprintf(stderr, "usage: %s OUTPUT [crosspolar]\n", argv[0]);
         return 2;
     }
+    const bool xpol = argc == 3 && std::string(argv[2]) == "crosspolar";
     const int nside_sky = 64, nside_beam = 512;
     const double deg = M_PI / 180.0;
     const long npix_beam = pisco_ang2pix(nside_beam, 4.0 * deg, 0.0);
@@ -54,7 +61,33 @@ int main(int argc, char **argv)
     Sky sky(nside_sky, I.data(), Q.data(), U.data(), V.data());
     Scan scan(n, ra.data(), dec.data(), pa.data());
     PolBeam beam(nside_beam, npix_beam, 0.01, 'p');
-    beam.make_unpol_gaussian_elliptical_beams(1.5, 1.4, 0.0);
+    std::vector<double> fields[8];   /* magEco, phsEco, magEcx, phsEcx of a, then b */
+    if (xpol) {
+        /* co-polar: sqrt of a Gaussian of 1.5 deg FWHM with a quadratic phase;
+         * cross-polar: 0.3 (a) or 0.2 (b) of it, modulated in sigma, with its
+         * own phase */
+        const double sig = 1.5 * deg / 2.35482;
+        for (auto &v : fields)
+            v.assign((size_t)npix_beam, 0.0);
+        for (long p = 0; p < npix_beam; p++) {
+            double th = 0.0, ph = 0.0;
+            pisco_pix2ang(nside_beam, p, &th, &ph);
+            const double u = th * th / (sig * sig);
+            const double co = std::exp(-0.25 * u);
+            for (int d = 0; d < 2; d++) {
+                fields[4 * d + 0][p] = co;
+                fields[4 * d + 1][p] = 0.2 * u;
+                fields[4 * d + 2][p] = (d ? 0.2 : 0.3) * co * (0.5 + 0.5 * std::cos(ph - d));
+                fields[4 * d + 3][p] = (d ? -0.8 : 0.5) + 0.4 * std::sin(ph);
+            }
+        }
+        beam.beam_from_fields('a', fields[0].data(), fields[1].data(), fields[2].data(),
+                              fields[3].data());
+        beam.beam_from_fields('b', fields[4].data(), fields[5].data(), fields[6].data(),
+                              fields[7].data());
+    } else {
+        beam.make_unpol_gaussian_elliptical_beams(1.5, 1.4, 0.0);
+    }
     beam.build(nside_sky);
 
     CUDACONV::RunConfig cfg{};
@@ -105,6 +138,9 @@ int main(int argc, char **argv)
     for (auto *v : {&ta, &tb, &ta2})
         std::fwrite(v->data(), sizeof(float), v->size(), f);
     std::fwrite(flags, sizeof flags, 1, f);
+    if (xpol)
+        for (auto &v : fields)
+            std::fwrite(v.data(), sizeof(double), v.size(), f);
     std::fclose(f);
     return 0;
 }

@@ -5,6 +5,9 @@ as a mismatch:
   - Convolver forward (nearest beam pixel and interpolation modes) of a
     detector pair, sky NSIDE 16, beam NSIDE 128 (cap 10 deg, FWHM 4 deg),
     300 raster samples, and the sky adjoint for fixed cotangents;
+  - the same with cross-polar beams (PolBeam::beam_from_fields of fields with
+    a cross-polar component, so every beam map is non-zero): forward, sky
+    adjoint and beam adjoint;
   - SkyMap accumulate + solve of 4 detectors x 2000 samples into NSIDE 8;
   - BPoint::focal_plane_to_sky of 5 detectors x 100 samples (ra/dec form).
 The oracle's own pins (tests/test_oracle_*.py) check these functions; the
@@ -43,6 +46,21 @@ def convolver_case():
                 c_dec=dec, c_pa=pa, c_g=g)
 
 
+def xconvolver_case(o):
+    """Cross-polar pair: fields from inputs.crosspolar_fields, beams built by
+    the oracle's beam_from_fields + build (polbeam.cpp:117-288)."""
+    ns, nb = 16, 128
+    I, Q, U, _ = inputs.iid_sky(ns, seed=111)
+    npb = pypisco.beam_npix_for_radius(nb, 10.0 * DEG)
+    fa = inputs.crosspolar_fields(nb, npb, 4.0, 0.1, 0.0, 0.35, 0.6, 0.3)
+    fb = inputs.crosspolar_fields(nb, npb, 4.0, 0.1, 30.0, 0.25, -0.9, 1.2)
+    beams = np.stack(o.polbeam_from_fields(npb, nb, ns, 0.01, fa, fb))
+    ra, dec, pa = inputs.raster_scan(300, size_deg=40.0, center_dec=0.4)
+    g = np.random.default_rng(112).normal(size=(2, 300)).astype(np.float32)
+    return dict(x_nside=ns, x_nside_beam=nb, x_I=I, x_Q=Q, x_U=U, x_beams=beams, x_ra=ra,
+                x_dec=dec, x_pa=pa, x_g=g)
+
+
 def skymap_case():
     rng = np.random.default_rng(103)
     ndet, n = 4, 2000
@@ -70,6 +88,14 @@ def outputs(o, f):
     gI, gQ, gU = o.adjoint_map(int(f["c_nside"]), int(f["c_nside_beam"]), beams, f["c_ra"],
                                f["c_dec"], f["c_pa"], f["c_g"][0], f["c_g"][1])
     out["c_gI"], out["c_gQ"], out["c_gU"] = gI, gQ, gU
+    xb = list(f["x_beams"])
+    xargs = (int(f["x_nside"]), f["x_I"], f["x_Q"], f["x_U"], int(f["x_nside_beam"]), xb,
+             f["x_ra"], f["x_dec"], f["x_pa"])
+    out["x_tod_a"], out["x_tod_b"] = o.convolve(*xargs, nthreads=1)
+    gI, gQ, gU = o.adjoint_map(int(f["x_nside"]), int(f["x_nside_beam"]), xb, f["x_ra"],
+                               f["x_dec"], f["x_pa"], f["x_g"][0], f["x_g"][1])
+    out["x_gI"], out["x_gQ"], out["x_gU"] = gI, gQ, gU
+    out["x_gbeam"] = o.adjoint_beam(*xargs, f["x_g"][0], f["x_g"][1])
     o.set_beam_interpolation(True)
     try:
         out["c_tod_a_interp"], out["c_tod_b_interp"] = o.convolve(*args, nthreads=1)
@@ -89,6 +115,7 @@ def main():
     o = oracle_bind.Oracle()
     f = {}
     f.update(convolver_case())
+    f.update(xconvolver_case(o))
     f.update(skymap_case())
     f.update(focal_case())
     f.update(outputs(o, f))

@@ -149,6 +149,28 @@ class Oracle:
                                 _parr(tilde), _parr(out))
         return out, tilde, f
 
+    def beam_from_fields(self, npix, magEco, phsEco, magEcx, phsEcx):
+        """tilde I, Q, U, V (float32) of one detector from its E fields."""
+        f = [np.ascontiguousarray(x, np.float64) for x in (magEco, phsEco, magEcx, phsEcx)]
+        tilde = [np.zeros(npix, np.float32) for _ in range(4)]
+        self.L.or_beam_from_fields(npix, *[_a(x) for x in f], *[_a(t) for t in tilde])
+        return tilde
+
+    def polbeam_build(self, npix, nside_beam, nside_sky, epsilon, dets, tilde):
+        """aBeams[0..3] + bBeams[0..3] from tilde Ia..Va, Ib..Vb."""
+        tilde = [np.ascontiguousarray(t, np.float32) for t in tilde]
+        out = [np.empty(npix, np.float32) for _ in range(8)]
+        self.L.or_polbeam_build(npix, nside_beam, nside_sky, epsilon, dets.encode(),
+                                _parr(tilde), _parr(out))
+        return out
+
+    def polbeam_from_fields(self, npix, nside_beam, nside_sky, epsilon, fields_a, fields_b):
+        """A detector pair built from E fields: beam_from_fields for a and b,
+        then build (polbeam.cpp:117-288)."""
+        tilde = (self.beam_from_fields(npix, *fields_a) +
+                 self.beam_from_fields(npix, *fields_b))
+        return self.polbeam_build(npix, nside_beam, nside_sky, epsilon, "p", tilde)
+
     # --- Convolver
     def convolve(self, nside_sky, I, Q, U, nside_beam, beams, ra, dec, pa, polflag="p",
                  precision64=False, nthreads=0):

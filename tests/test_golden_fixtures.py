@@ -17,11 +17,22 @@ def fx():
     return dict(np.load(FIX))
 
 
+def test_crosspolar_fixture_inputs(oracle, fx):
+    """The cross-polar case's beams are what inputs.crosspolar_fields and the
+    oracle's PolBeam give, and they exercise the A2/B2 terms."""
+    import make_oracle_fixtures as m
+    x = m.xconvolver_case(oracle)
+    assert np.array_equal(x["x_beams"], fx["x_beams"])
+    b = fx["x_beams"]
+    assert np.abs(b[2]).max() >= 0.1 * np.abs(b[1]).max()
+    assert np.abs(b[6]).max() >= 0.1 * np.abs(b[5]).max()
+
+
 def test_oracle_reproduces_its_fixtures(oracle, fx):
     import make_oracle_fixtures as m
     out = m.outputs(oracle, fx)
-    for k in ("c_tod_a", "c_tod_b", "c_tod_a_interp", "c_tod_b_interp", "m_AtA", "m_AtD",
-              "m_I", "m_Q", "m_U", "m_hits", "f_xdet", "f_ydet", "f_rotdet"):
+    for k in ("c_tod_a", "c_tod_b", "c_tod_a_interp", "c_tod_b_interp", "x_tod_a", "x_tod_b",
+              "m_AtA", "m_AtD", "m_I", "m_Q", "m_U", "m_hits", "f_xdet", "f_ydet", "f_rotdet"):
         assert np.array_equal(out[k], fx[k]), k
-    for k in ("c_gI", "c_gQ", "c_gU"):
+    for k in ("c_gI", "c_gQ", "c_gU", "x_gI", "x_gQ", "x_gU", "x_gbeam"):
         assert np.allclose(out[k], fx[k], rtol=1e-12, atol=1e-12 * np.abs(fx[k]).max()), k

@@ -1,6 +1,8 @@
 """The LDS-privatised sky adjoint: against the FP64 oracle, against the
 direct-atomic mode, with windows forced to overflow (fallback path), and the
-dot-product identity at the headline geometry (NSIDE 512, 5 deg cap)."""
+dot-product identity at the headline geometry (NSIDE 512, 5 deg cap).
+Every case runs on a co-polar Gaussian beam and on a cross-polar beam (all
+beam maps non-zero, so the A2/B2 terms of the adjoint are compared)."""
 import math
 
 import numpy as np
@@ -26,11 +28,14 @@ def conv_for(nside_sky, sky, beam, ra, dec, pa):
     return conv
 
 
-@pytest.fixture(scope="module")
-def c3_small():
+KINDS = ["copolar", "crosspolar"]
+
+
+@pytest.fixture(scope="module", params=KINDS)
+def c3_small(request):
     nside_sky = 512
     sky = inputs.iid_sky(nside_sky, seed=3)
-    beam = inputs.gaussian_polbeam(4096, 5.0, 0.33, 0.02, 0.01, nside_sky=nside_sky)
+    beam = inputs.make_polbeam(request.param, 4096, 5.0, 0.33, 0.02, 0.01, nside_sky=nside_sky)
     ra, dec, pa = inputs.ces_scan(1500, fs=100.0)
     return nside_sky, sky, beam, ra, dec, pa
 
@@ -73,13 +78,14 @@ def test_adjoint_lds_shapes_and_fallback(c3_small, oracle, chunk, cap):
         assert rel_rms(got[k], ref[k]) < 1e-6
 
 
-def test_auto_chunk_small_discs(oracle):
+@pytest.mark.parametrize("kind", KINDS)
+def test_auto_chunk_small_discs(oracle, kind):
     """Small discs (sky NSIDE 256, cap 1.5 deg): the default chunk is chosen
     from the disc size (128 samples per window here); the result equals the
     oracle and the 16-sample windows within relative RMS 1e-6."""
     nside_sky = 256
     sky = inputs.iid_sky(nside_sky, seed=31)
-    beam = inputs.gaussian_polbeam(2048, 1.5, 0.5, 0.03, 0.01, nside_sky=nside_sky)
+    beam = inputs.make_polbeam(kind, 2048, 1.5, 0.5, 0.03, 0.01, nside_sky=nside_sky)
     ra, dec, pa = inputs.ces_scan(20000, t0=300.0)
     rng = np.random.default_rng(32)
     ga, gb = (rng.normal(size=ra.size).astype(np.float32) for _ in range(2))
@@ -95,12 +101,13 @@ def test_auto_chunk_small_discs(oracle):
         assert rel_rms(auto[k], c16[k]) < 1e-6
 
 
-def test_adjoint_polar_and_wraparound(oracle):
+@pytest.mark.parametrize("kind", KINDS)
+def test_adjoint_polar_and_wraparound(oracle, kind):
     """Pointings through both poles and across ra = 0 (ring windows that wrap
     and whole polar rings)."""
     nside_sky = 128
     sky = inputs.iid_sky(nside_sky, seed=4)
-    beam = inputs.gaussian_polbeam(1024, 4.0, 1.5, 0.0, 0.0, nside_sky=nside_sky)
+    beam = inputs.make_polbeam(kind, 1024, 4.0, 1.5, 0.0, 0.0, nside_sky=nside_sky)
     n = 2000
     t = np.linspace(0, 2 * math.pi, n)
     dec = (math.pi / 2 * np.sin(t)).astype(np.float32)      # meridian pole to pole

@@ -2,8 +2,10 @@
 forward and the CPU oracle: per pair, the batched TOD is the TOD of that pair
 with its own pa (pa + pa_offset in float).  Pairs have distinct beams, disc
 radii (cap 3 x FWHM), beam NSIDEs and position-angle offsets; 7 pairs
-exercise the padding of the table to a multiple of the group size.
-Tolerance: relative RMS 1e-6 per detector."""
+exercise the padding of the table to a multiple of the group size; pairs
+alternate between co-polar and cross-polar beams (all beam maps non-zero).
+Each DetectorBatch has a context of its own, so launch shapes are set on the
+batch.  Tolerance: relative RMS 1e-6 per detector."""
 import math
 
 import numpy as np
@@ -12,7 +14,7 @@ import torch
 
 from pisco4cmb_amd import inputs, torch_ops
 from pisco4cmb_amd.batched import DetectorBatch
-from pisco4cmb_amd._lib import check, lib
+from pisco4cmb_amd._lib import PISCO_ERR_STATE, PiscoError
 
 pytestmark = pytest.mark.gpu
 NS = 128
@@ -32,8 +34,8 @@ def case():
     for k in range(7):
         fwhm = float(rng.uniform(0.6, 2.0))
         nb = 512 if k % 3 else 1024
-        beam = inputs.gaussian_polbeam(nb, 3 * fwhm, fwhm, float(rng.uniform(0, 0.1)), 0.01,
-                                       nside_sky=NS)
+        beam = inputs.make_polbeam("crosspolar" if k % 2 == 0 else "copolar", nb, 3 * fwhm,
+                                   fwhm, float(rng.uniform(0, 0.1)), 0.01, nside_sky=NS)
         pairs.append((beam, nb, float(rng.uniform(-math.pi, math.pi))))
     ra, dec, pa = inputs.raster_scan(1500)
     return sky, pairs, ra, dec, pa
@@ -50,8 +52,7 @@ def test_batched_forward_matches_single_pair_and_oracle(case, oracle, group):
     rat, dect, pat = t(ra), t(dec), t(pa)
     specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
     batch = DetectorBatch(specs)
-    ctx = torch_ops._ctx(torch.device("cuda:0"))
-    check(lib.pisco_set_tuning(ctx, b"batch_group", group))
+    batch.set_tuning("batch_group", group)
     tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
     for k, (b, nb, off) in enumerate(pairs):
         pa_k = (pa + np.float32(off)).astype(np.float32)
@@ -63,7 +64,6 @@ def test_batched_forward_matches_single_pair_and_oracle(case, oracle, group):
             ra_, rb_ = oracle.convolve(NS, *sky[:3], nb, list(b.aBeams) + list(b.bBeams),
                                        ra, dec, pa_k)
             assert rel_rms(tod[k, 0], ra_) < 1e-6 and rel_rms(tod[k, 1], rb_) < 1e-6
-    check(lib.pisco_set_tuning(ctx, b"batch_group", 4))
     batch.close()
 
 
@@ -78,8 +78,7 @@ def test_batched_adjoint_sum_of_pairs_and_dot_product(case, adj_group):
     rat, dect, pat = t(ra), t(dec), t(pa)
     specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
     batch = DetectorBatch(specs)
-    ctx = torch_ops._ctx(torch.device("cuda:0"))
-    check(lib.pisco_set_tuning(ctx, b"batch_adj_group", adj_group))
+    batch.set_tuning("batch_adj_group", adj_group)
     rng = np.random.default_rng(8)
     g = t(rng.normal(size=(len(pairs), 2, ra.size)).astype(np.float32))
     tod = batched_convolve(I, Q, U, batch, rat, dect, pat, NS)
@@ -100,7 +99,6 @@ def test_batched_adjoint_sum_of_pairs_and_dot_product(case, adj_group):
     lhs = float(L.item())
     rhs = sum(float(np.dot(got[c].astype(np.float64), sky[c])) for c in range(3))
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
-    check(lib.pisco_set_tuning(ctx, b"batch_adj_group", 4))
     batch.close()
 
 
@@ -138,15 +136,12 @@ def test_autotuner_picks_a_valid_shape(case):
     I, Q, U = (t(x) for x in sky[:3])
     specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
     batch = DetectorBatch(specs)
+    before = tuning()
     res = autotune_batch(batch, I, Q, U, t(ra), t(dec), t(pa), NS, sample=1024, reps=1)
-    cur = tuning()
+    cur = tuning(batch=batch)
     for key, (best, times) in res.items():
         assert best in times and cur[key] == best
-    from pisco4cmb_amd._lib import check as _check
-    ctx = torch_ops._ctx(torch.device("cuda:0"))
-    for k, v in (("batch_group", 4), ("batch_adj_group", 4), ("adjmap_chunk", -1),
-                 ("batch_pair_major", -1), ("batch_pair_range", 0)):
-        _check(lib.pisco_set_tuning(ctx, k.encode(), v))
+    assert tuning() == before   # single-pair calls keep their shapes
     batch.close()
 
 
@@ -164,17 +159,13 @@ def test_pair_major_matches_interleaved(case, per_pair):
     rat, dect, pat = t(ra), t(dec), t(pa)
     specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
     batch = DetectorBatch(specs)
-    ctx = torch_ops._ctx(torch.device("cuda:0"))
     g = torch.randn((len(pairs), 2, ra.shape[-1]), device="cuda")
     res = {}
-    try:
-        for mode in (0, 1):
-            check(lib.pisco_set_tuning(ctx, b"batch_pair_major", mode))
-            tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
-            adj = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
-            res[mode] = (tod, adj)
-    finally:
-        check(lib.pisco_set_tuning(ctx, b"batch_pair_major", -1))
+    for mode in (0, 1):
+        batch.set_tuning("batch_pair_major", mode)
+        tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
+        adj = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
+        res[mode] = (tod, adj)
     for k in range(len(pairs)):
         for d in range(2):
             assert rel_rms(res[1][0][k, d], res[0][0][k, d]) < 1e-6
@@ -192,9 +183,9 @@ def test_pair_major_auto_for_large_beams(oracle):
     sky = inputs.iid_sky(ns, seed=71)
     I, Q, U = (t(x) for x in sky[:3])
     rng = np.random.default_rng(72)
-    beams = [inputs.gaussian_polbeam(4096, 5.0, float(rng.uniform(0.3, 0.36)),
-                                     float(rng.uniform(0, 0.1)), 0.01, nside_sky=ns)
-             for _ in range(2)]
+    beams = [inputs.make_polbeam(kind, 4096, 5.0, float(rng.uniform(0.3, 0.36)),
+                                 float(rng.uniform(0, 0.1)), 0.01, nside_sky=ns)
+             for kind in ("crosspolar", "copolar")]
     offs = [0.4, -1.1]
     ra, dec, pa = inputs.ces_scan(1200, t0=900.0)
     rat, dect, pat = t(ra), t(dec), t(pa)
@@ -233,27 +224,71 @@ def test_pair_ranges(case):
     rat, dect, pat = t(ra), t(dec), t(pa)
     specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
     batch = DetectorBatch(specs)
-    ctx = torch_ops._ctx(torch.device("cuda:0"))
     g = torch.randn((len(pairs), 2, ra.size), device="cuda")
     res, det = {}, {}
-    try:
-        check(lib.pisco_set_tuning(ctx, b"batch_pair_major", 0))
-        for rng_ in (-1, 4, 0):
-            check(lib.pisco_set_tuning(ctx, b"batch_pair_range", rng_))
-            tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
-            adj = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
-            res[rng_] = (tod, adj)
-            check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 1))
-            det[rng_] = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
-            check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 0))
-    finally:
-        for k, v in ((b"batch_pair_range", 0), (b"batch_pair_major", -1),
-                     (b"adjmap_deterministic", 0)):
-            check(lib.pisco_set_tuning(ctx, k, v))
+    batch.set_tuning("batch_pair_major", 0)
+    for rng_ in (-1, 4, 0):
+        batch.set_tuning("batch_pair_range", rng_)
+        tod = batch.forward(I, Q, U, rat, dect, pat, NS).cpu().numpy()
+        adj = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
+        res[rng_] = (tod, adj)
+        batch.set_tuning("adjmap_deterministic", 1)
+        det[rng_] = [x.cpu().numpy() for x in batch.adjoint_map(g, rat, dect, pat, NS)]
+        batch.set_tuning("adjmap_deterministic", 0)
     for rng_ in (4, 0):
         assert rel_rms(res[rng_][0], res[-1][0]) < 1e-6
         for a, b in zip(res[rng_][1], res[-1][1]):
             assert rel_rms(a, b) < 1e-6
         for a, b in zip(det[rng_], det[-1]):
             assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    batch.close()
+
+
+def test_graph_survives_larger_calls(case):
+    """A captured step keeps replaying to the same bits when, after capture,
+    (1) a single-pair call on the same device needs more workspace than the
+    batch's calls did (it runs on another context), and (2) the batch itself
+    is called on a larger sky (NSIDE 256 against the captured 128): that
+    call is refused (PISCO_ERR_STATE), as its workspace is frozen for the
+    graph, and works again once unfrozen.  The sky adjoint runs in its
+    deterministic mode, whose bits do not depend on the order of the LDS
+    sums, so replays are compared bit for bit."""
+    from pisco4cmb_amd.batched import BatchedStepGraph
+    sky, pairs, ra, dec, pa = case
+    n = 300
+    I, Q, U = (t(x) for x in sky[:3])
+    rat, dect, pat = t(ra[:n]), t(dec[:n]), t(pa[:n])
+    specs = [(t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), nb, off) for b, nb, off in pairs]
+    batch = DetectorBatch(specs)
+    batch.set_tuning("adjmap_deterministic", 1)
+    g = torch.randn((len(pairs), 2, n), device="cuda")
+    step = BatchedStepGraph(batch, I, Q, U, rat, dect, pat, g, NS)
+    tod, grad = step.replay()
+    torch.cuda.synchronize()
+    ref = [tod.cpu().numpy()] + [x.cpu().numpy() for x in grad]
+    # a larger single-pair call on the same device: sky NSIDE 512, beam NSIDE 4096
+    ns2 = 512
+    big = inputs.gaussian_polbeam(4096, 5.0, 0.33, 0.0, 0.0, nside_sky=ns2)
+    sky2 = [t(x) for x in inputs.iid_sky(ns2, seed=9)[:3]]
+    r2, d2, p2 = (t(x) for x in inputs.ces_scan(4000))
+    for x in sky2:
+        x.requires_grad_(True)
+    ta, tb = torch_ops.beam_convolve(*sky2, t(np.stack(big.aBeams)), t(np.stack(big.bBeams)),
+                                     r2, d2, p2, ns2, 4096)
+    (ta.sum() + tb.sum()).backward()
+    # the batch on a larger sky than captured
+    with pytest.raises(PiscoError) as e:
+        batch.adjoint_map(g, rat, dect, pat, 2 * NS)
+    assert e.value.code == PISCO_ERR_STATE
+    for _ in range(3):
+        tod, grad = step.replay()
+        torch.cuda.synchronize()
+        got = [tod.cpu().numpy()] + [x.cpu().numpy() for x in grad]
+        for a, b in zip(got, ref):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    del step
+    batch.freeze(False)
+    out = batch.adjoint_map(g, rat, dect, pat, 2 * NS)
+    assert all(bool(torch.isfinite(x).all()) for x in out)
+    assert float(out[0].abs().sum()) > 0
     batch.close()

@@ -4,7 +4,9 @@ against the oracle's FP64 beam adjoint and against the global-atomic form
 (adjbeam_bands = 0), relative RMS 1e-6 per column: the headline beam shape
 (81 bands), a cap reaching the equatorial rings of a coarse beam map (bands
 of 300 rows: whole and partial rings), small and large work items, samples
-near the poles; and the dot product with the forward."""
+near the poles; and the dot product with the forward.  The headline shape and
+the dot product run on co-polar and cross-polar beams (the gradient's
+columns do not depend on the beam values, the dot product does)."""
 import math
 
 import numpy as np
@@ -53,10 +55,11 @@ def check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, chunks=(1024,), row
     return conv, ga, gb, res
 
 
-def test_headline_beam_shape(oracle):
+@pytest.mark.parametrize("kind", ["copolar", "crosspolar"])
+def test_headline_beam_shape(oracle, kind):
     ns = 512
     sky = inputs.iid_sky(ns, seed=82)
-    beam = inputs.gaussian_polbeam(4096, 5.0, 0.33, 0.05, 0.01, nside_sky=ns)
+    beam = inputs.make_polbeam(kind, 4096, 5.0, 0.33, 0.05, 0.01, nside_sky=ns)
     ra, dec, pa = inputs.ces_scan(400, t0=1300.0)
     check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, chunks=(1024, 7))
     check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, rows=1000)
@@ -79,12 +82,13 @@ def test_coarse_beam_equatorial_rings_and_poles(oracle):
     check_against_oracle(oracle, ns, sky, b, ra, dec, pa)
 
 
-def test_dot_product(oracle):
+@pytest.mark.parametrize("kind", ["copolar", "crosspolar"])
+def test_dot_product(oracle, kind):
     """<TOD, g> = <beams, A_beam^T g>, with g = TOD + noise so that the inner
     product does not cancel."""
     ns = 256
     sky = inputs.iid_sky(ns, seed=85)
-    beam = inputs.gaussian_polbeam(2048, 1.5, 0.5, 0.05, 0.01, nside_sky=ns)
+    beam = inputs.make_polbeam(kind, 2048, 1.5, 0.5, 0.05, 0.01, nside_sky=ns)
     ra, dec, pa = inputs.ces_scan(5000, t0=77.0)
     conv = conv_for(ns, sky, beam, ra, dec, pa)
     a = np.zeros(ra.size, np.float32)
@@ -122,3 +126,48 @@ def test_small_beams_and_one_cotangent(oracle, npix, rows):
         for k in range(3):
             assert rel_rms(oa[k], G[:, k]) < 1e-6
         assert np.all(ob[0] == 0) and np.all(ob[1] == 0) and np.all(ob[2] == 0)
+
+
+SMALL_ROWS_FIRST = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from pisco4cmb_amd import inputs, pypisco
+ns = 128
+sky = inputs.iid_sky(ns, seed=89)
+ra, dec, pa = inputs.ces_scan(600, t0=40.0)
+g = np.random.default_rng(90).normal(size=(2, ra.size)).astype(np.float32)
+out = {}
+for npb_deg, nb, rows in ((3.0, 256, 300), (3.0, 1024, 0)):
+    beam = inputs.gaussian_polbeam(nb, npb_deg, 1.0, 0.05, 0.01, nside_sky=ns)
+    conv = pypisco.GPUConvolver(pypisco.default_run_config())
+    conv.configure_execution()
+    conv.update_sky(pypisco.Sky(ns, *sky))
+    conv.update_beam(beam)
+    conv.update_scan(pypisco.Scan(ra.size, ra, dec, pa))
+    conv.set_tuning("adjbeam_rows", rows)
+    banded = conv.exec_adjoint_beam(g[0], g[1])
+    conv.set_tuning("adjbeam_bands", 0)
+    atomic = conv.exec_adjoint_beam(g[0], g[1])
+    for x, y in zip(banded[0][:3] + banded[1][:3], atomic[0][:3] + atomic[1][:3]):
+        x = x.astype(np.float64); y = y.astype(np.float64)
+        assert np.sqrt(np.mean((x - y) ** 2)) <= 1e-6 * np.sqrt(np.mean(y ** 2)), (nb, rows)
+    print("rows", rows, "beam rows", beam.nPixels, "ok", flush=True)
+    conv.close()
+print("SMALL_ROWS_FIRST_OK", flush=True)
+"""
+
+
+def test_small_band_rows_first_in_a_fresh_process():
+    """A process whose first banded beam adjoint uses small bands (300 rows)
+    and whose second uses the default bands of an 8,580-row beam: the second
+    launch needs a larger dynamic-LDS limit than the first set, and must
+    raise it (both equal the global-atomic form)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", SMALL_ROWS_FIRST, root], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "SMALL_ROWS_FIRST_OK" in r.stdout

@@ -4,7 +4,9 @@
 pair with its own ellipticity and position-angle offset, on one CES scan.
 The batched forward equals the single-pair forward of every pair and the
 CPU oracle on a prefix of the samples (relative RMS 1e-6); the batched sky
-adjoint equals the sum of the single-pair adjoints (relative RMS 1e-6)."""
+adjoint equals the sum of the single-pair adjoints (relative RMS 1e-6).
+The first pair of each family has cross-polar beams (all maps non-zero); the
+oracle checks those."""
 import math
 
 import numpy as np
@@ -36,9 +38,9 @@ def case():
     rng = np.random.default_rng(57)
     pairs = []
     for fwhm, nb in FAMILIES:
-        for _ in range(2):
-            b = inputs.gaussian_polbeam(nb, 3 * fwhm, fwhm, float(rng.uniform(0, 0.1)), 0.01,
-                                        nside_sky=NS)
+        for kind in ("crosspolar", "copolar"):
+            b = inputs.make_polbeam(kind, nb, 3 * fwhm, fwhm, float(rng.uniform(0, 0.1)), 0.01,
+                                    nside_sky=NS)
             pairs.append((b, nb, float(rng.uniform(-math.pi, math.pi))))
     ra, dec, pa = inputs.ces_scan(2000, t0=500.0)
     return (I, Q, U), pairs, ra, dec, pa

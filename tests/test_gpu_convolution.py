@@ -1,5 +1,9 @@
 """TOD and adjoint parity of the HIP path against the CPU oracle.
 
+Every family runs on co-polar Gaussian beams (aBeams[2] = bBeams[2] = 0)
+and on cross-polar beams (inputs.crosspolar_polbeam: all beam maps
+non-zero), so the A2/B2 terms of convolver.cpp:157-164 are compared too.
+
 Tolerances: TOD within 1e-6 relative RMS of the reference-faithful oracle
 (float products, FP64 accumulation, convolver.cpp:157-164); the error against
 the full-FP64 oracle is asserted too.  Adjoints against the oracle's FP64
@@ -37,12 +41,15 @@ def beams_of(b):
     return list(b.aBeams) + list(b.bBeams)
 
 
-@pytest.fixture(scope="module")
-def c1():
+KINDS = ["copolar", "crosspolar"]
+
+
+@pytest.fixture(scope="module", params=KINDS)
+def c1(request):
     """Config 1: NSIDE 128 iid sky, beam NSIDE 1024, cap 2.5 deg, FWHM 1 deg,
     ellipticity 0.05, leakage 1%, 1e4 raster samples, pa cycling."""
     sky = inputs.iid_sky(128, seed=1)
-    beam = inputs.gaussian_polbeam(1024, 2.5, 1.0, 0.05, 0.01, nside_sky=128)
+    beam = inputs.make_polbeam(request.param, 1024, 2.5, 1.0, 0.05, 0.01, nside_sky=128)
     ra, dec, pa = inputs.raster_scan(10000)
     return sky, beam, ra, dec, pa
 
@@ -89,12 +96,13 @@ def test_forward_chunking_and_tail(c1, oracle):
     assert rel_rms(a1[-1:], ra_[-1:]) < 1e-6
 
 
+@pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("nside_sky,nside_beam,cap,fwhm,n", [(256, 2048, 1.5, 0.5, 4000),
                                                              (512, 4096, 5.0, 0.33, 300),
                                                              (64, 64, 10.0, 3.0, 3000)])
-def test_forward_other_configs(oracle, nside_sky, nside_beam, cap, fwhm, n):
+def test_forward_other_configs(oracle, nside_sky, nside_beam, cap, fwhm, n, kind):
     sky = inputs.iid_sky(nside_sky, seed=nside_sky)
-    beam = inputs.gaussian_polbeam(nside_beam, cap, fwhm, 0.05, 0.01, nside_sky=nside_sky)
+    beam = inputs.make_polbeam(kind, nside_beam, cap, fwhm, 0.05, 0.01, nside_sky=nside_sky)
     ra, dec, pa = inputs.ces_scan(n, fs=2.0)
     _, a, b = run_gpu(nside_sky, sky, beam, ra, dec, pa)
     ra_, rb_ = oracle.convolve(nside_sky, *sky[:3], nside_beam, beams_of(beam), ra, dec, pa)
@@ -141,8 +149,9 @@ def test_adjoint_beam_matches_oracle_and_dot_product(c1, oracle):
     assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), 1e-30)
 
 
+@pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("polflag", ["p", "a", "b"])
-def test_host_api_split_copies(oracle, polflag):
+def test_host_api_split_copies(oracle, polflag, kind):
     """From 2^18 samples the host-API forward runs in 4 launches whose TOD
     copies overlap the next launch: 2^18 + 7 samples (a ragged last part)
     give the bits of the device-resident forward (one launch) for every
@@ -152,7 +161,7 @@ def test_host_api_split_copies(oracle, polflag):
     from pisco4cmb_amd import torch_ops
     ns, nb, n = 64, 512, (1 << 18) + 7
     sky = inputs.iid_sky(ns, seed=71)
-    beam = inputs.gaussian_polbeam(nb, 3.0, 1.2, 0.05, 0.01, nside_sky=ns)
+    beam = inputs.make_polbeam(kind, nb, 3.0, 1.2, 0.05, 0.01, nside_sky=ns)
     ra, dec, pa = inputs.ces_scan(n, t0=300.0)
     t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
     da, db = (x.cpu().numpy() for x in torch_ops.beam_convolve(
@@ -166,14 +175,15 @@ def test_host_api_split_copies(oracle, polflag):
     assert rel_rms(da[k], oa) < 1e-6 and rel_rms(db[k], ob) < 1e-6
 
 
-def test_host_api_split_copies_interpolation(oracle):
+@pytest.mark.parametrize("kind", KINDS)
+def test_host_api_split_copies_interpolation(oracle, kind):
     """The same 4-launch host-API forward in beam interpolation mode: 2^18 + 7
     samples, whose last 3,000 equal a host-API call over those samples alone
     (one launch) bit for bit, and whose last 300 match the oracle's
     interpolated forward."""
     ns, nb, n = 64, 512, (1 << 18) + 7
     sky = inputs.iid_sky(ns, seed=73)
-    beam = inputs.gaussian_polbeam(nb, 4.0, 1.5, 0.08, 0.02, nside_sky=ns)
+    beam = inputs.make_polbeam(kind, nb, 4.0, 1.5, 0.08, 0.02, nside_sky=ns)
     ra, dec, pa = inputs.ces_scan(n, t0=900.0)
 
     def run(r, d, q):

@@ -3,7 +3,8 @@ reference's class surface (tests/cpp/drop_in_driver.cpp, built by
 pisco4cmb_amd.build) runs GPUConvolver and Convolver; its TOD is compared
 with the CPU oracle on the very inputs it wrote, and its error behaviour
 (std::invalid_argument for a bad polFlag and an invalid RunConfig) is
-checked."""
+checked.  It runs with co-polar Gaussian beams and with beams the C++
+PolBeam built from cross-polar fields (all beam maps non-zero)."""
 import os
 import subprocess
 
@@ -22,7 +23,7 @@ def rel_rms(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
 
 
-def read_output(path):
+def read_output(path, crosspolar=False):
     raw = open(path, "rb").read()
     ns, nb = np.frombuffer(raw, np.int32, 2, 0)
     npb, n = np.frombuffer(raw, np.int64, 2, 8)
@@ -44,17 +45,23 @@ def read_output(path):
     for k in ("ta", "tb", "ta_convolver"):
         take(k, n)
     take("flags", 2, np.int32)
+    if crosspolar:
+        for k in range(8):
+            take("field%d" % k, npb, np.float64)
     assert off == len(raw)
     return int(ns), int(nb), out
 
 
-def test_cpp_drop_in_driver(oracle, tmp_path):
+@pytest.mark.parametrize("kind", ["copolar", "crosspolar"])
+def test_cpp_drop_in_driver(oracle, tmp_path, kind):
     if not os.path.exists(DRIVER):
         pytest.fail("tests/cpp/drop_in_driver is not built (python -m pisco4cmb_amd.build)")
     path = str(tmp_path / "drop_in.bin")
-    r = subprocess.run([DRIVER, path], capture_output=True, text=True, timeout=120)
+    xpol = kind == "crosspolar"
+    r = subprocess.run([DRIVER, path] + (["crosspolar"] if xpol else []), capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    ns, nb, o = read_output(path)
+    ns, nb, o = read_output(path, xpol)
     beams = [o["beam%d" % k] for k in range(8)]
     ra_, rb_ = oracle.convolve(ns, o["I"], o["Q"], o["U"], nb, beams, o["ra"], o["dec"], o["pa"])
     assert rel_rms(o["ta"], ra_) < 1e-6 and rel_rms(o["tb"], rb_) < 1e-6
@@ -62,6 +69,12 @@ def test_cpp_drop_in_driver(oracle, tmp_path):
     assert list(o["flags"]) == [1, 1]
     # the beams the C++ PolBeam built are the oracle's PolBeam
     npb = beams[0].size
-    ref_beams, _, _ = oracle.gaussian_polbeam(nb, npb, 1.5, 1.4, 0.0, 0.01, "p", ns)
+    if xpol:
+        f = [o["field%d" % k] for k in range(8)]
+        ref_beams = oracle.polbeam_from_fields(npb, nb, ns, 0.01, f[:4], f[4:])
+        assert np.abs(beams[2]).max() >= 0.1 * np.abs(beams[1]).max()
+        assert np.abs(beams[6]).max() >= 0.1 * np.abs(beams[5]).max()
+    else:
+        ref_beams, _, _ = oracle.gaussian_polbeam(nb, npb, 1.5, 1.4, 0.0, 0.01, "p", ns)
     for k in range(8):
         assert np.array_equal(beams[k], ref_beams[k])

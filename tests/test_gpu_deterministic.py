@@ -47,9 +47,10 @@ def adjoint(ctx, beam8, nb, ra, dec, pa, ga, gb, ns):
     return [o.cpu().numpy() for o in out]
 
 
-def test_single_pair_bits_independent_of_launch_shape(ctx, oracle):
+@pytest.mark.parametrize("kind", ["copolar", "crosspolar"])
+def test_single_pair_bits_independent_of_launch_shape(ctx, oracle, kind):
     ns, nb = 128, 1024
-    beam = inputs.gaussian_polbeam(nb, 3.0, 1.0, 0.05, 0.01, nside_sky=ns)
+    beam = inputs.make_polbeam(kind, nb, 3.0, 1.0, 0.05, 0.01, nside_sky=ns)
     beam8 = torch_ops.pack_beam(t(np.stack(beam.aBeams)), t(np.stack(beam.bBeams)))
     ra, dec, pa = inputs.ces_scan(6000, t0=200.0)
     rng = np.random.default_rng(41)
@@ -83,13 +84,14 @@ def test_single_pair_bits_independent_of_launch_shape(ctx, oracle):
 
 
 @pytest.mark.parametrize("per_pair", [False, True])
-def test_batched_bits_independent_of_launch_shape(ctx, per_pair):
+def test_batched_bits_independent_of_launch_shape(per_pair):
     ns = 128
     rng = np.random.default_rng(43)
     specs = []
     for k in range(5):
         fwhm = float(rng.uniform(0.6, 1.5))
-        b = inputs.gaussian_polbeam(512, 3 * fwhm, fwhm, 0.05, 0.01, nside_sky=ns)
+        b = inputs.make_polbeam("crosspolar" if k % 2 else "copolar", 512, 3 * fwhm, fwhm, 0.05,
+                                0.01, nside_sky=ns)
         specs.append((t(np.stack(b.aBeams)), t(np.stack(b.bBeams)), 512,
                       float(rng.uniform(-1, 1))))
     batch = DetectorBatch(specs)
@@ -99,11 +101,11 @@ def test_batched_bits_independent_of_launch_shape(ctx, per_pair):
                        (ra, dec, pa))
     g = torch.randn((5, 2, 3000), device="cuda")
     ref = [x.cpu().numpy() for x in batch.adjoint_map(g, t(ra), t(dec), t(pa), ns)]
-    check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 1))
+    batch.set_tuning("adjmap_deterministic", 1)
     runs = []
     for blocks, chunk in ((0, 16), (131, 8), (0, 16)):
-        check(lib.pisco_set_tuning(ctx, b"adjmap_blocks", blocks))
-        check(lib.pisco_set_tuning(ctx, b"adjmap_chunk", chunk))
+        batch.set_tuning("adjmap_blocks", blocks)
+        batch.set_tuning("adjmap_chunk", chunk)
         runs.append([x.cpu().numpy() for x in batch.adjoint_map(g, t(ra), t(dec), t(pa), ns)])
     for r in runs[1:]:
         for a, b in zip(r, runs[0]):

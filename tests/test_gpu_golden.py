@@ -1,7 +1,8 @@
 """The GPU paths against the committed oracle fixtures
 (tests/golden/oracle_small.npz, inputs included): forward TOD in the
 nearest-pixel and interpolation modes and the sky adjoint within relative
-RMS 1e-6, SkyMap sums and maps and the focal-plane pointing bit for bit."""
+RMS 1e-6, the cross-polar case's forward and both adjoints likewise, SkyMap
+sums and maps and the focal-plane pointing bit for bit."""
 import os
 
 import numpy as np
@@ -60,6 +61,30 @@ def test_convolver_fixture(fx, interp):
         gI, gQ, gU = conv.exec_adjoint_map(fx["c_g"][0], fx["c_g"][1])
         for g, k in ((gI, "c_gI"), (gQ, "c_gQ"), (gU, "c_gU")):
             assert rel_rms(g, fx[k]) < 1e-6
+    conv.close()
+
+
+def test_crosspolar_convolver_fixture(fx):
+    """Cross-polar beams (every beam map non-zero): forward TOD, sky adjoint
+    and beam adjoint against the frozen oracle outputs (relative RMS 1e-6)."""
+    ns, nb = int(fx["x_nside"]), int(fx["x_nside_beam"])
+    n = fx["x_ra"].size
+    conv = pypisco.GPUConvolver(pypisco.default_run_config())
+    conv.configure_execution()
+    conv.update_sky(pypisco.Sky(ns, fx["x_I"], fx["x_Q"], fx["x_U"]))
+    conv.update_beam(_Beam(nb, fx["x_beams"]))
+    conv.update_scan(pypisco.Scan(n, fx["x_ra"], fx["x_dec"], fx["x_pa"]))
+    ta = np.zeros(n, np.float32)
+    tb = np.zeros(n, np.float32)
+    conv.exec_convolution(ta, tb)
+    assert rel_rms(ta, fx["x_tod_a"]) < 1e-6 and rel_rms(tb, fx["x_tod_b"]) < 1e-6
+    gI, gQ, gU = conv.exec_adjoint_map(fx["x_g"][0], fx["x_g"][1])
+    for g, k in ((gI, "x_gI"), (gQ, "x_gQ"), (gU, "x_gU")):
+        assert rel_rms(g, fx[k]) < 1e-6
+    oa, ob = conv.exec_adjoint_beam(fx["x_g"][0], fx["x_g"][1])
+    for k in range(3):
+        assert rel_rms(oa[k], fx["x_gbeam"][:, k]) < 1e-6
+        assert rel_rms(ob[k], fx["x_gbeam"][:, 3 + k]) < 1e-6
     conv.close()
 
 

@@ -28,12 +28,13 @@ def interp_oracle(oracle):
     oracle.set_beam_interpolation(False)
 
 
+@pytest.mark.parametrize("kind", ["copolar", "crosspolar"])
 @pytest.mark.parametrize("nside,nside_beam,cap,fwhm,n", [(64, 512, 4.0, 1.5, 3000),
                                                         (128, 1024, 2.0, 0.7, 3000)])
-def test_interpolated_forward_and_adjoints(interp_oracle, nside, nside_beam, cap, fwhm, n):
+def test_interpolated_forward_and_adjoints(interp_oracle, nside, nside_beam, cap, fwhm, n, kind):
     o = interp_oracle
     sky = inputs.iid_sky(nside, seed=31)
-    beam = inputs.gaussian_polbeam(nside_beam, cap, fwhm, 0.08, 0.02, nside_sky=nside)
+    beam = inputs.make_polbeam(kind, nside_beam, cap, fwhm, 0.08, 0.02, nside_sky=nside)
     ra, dec, pa = inputs.raster_scan(n, size_deg=15.0)
     conv = pypisco.GPUConvolver(pypisco.default_run_config(ptgPerConv=1000))
     conv.configure_execution()
@@ -73,20 +74,16 @@ def test_interpolated_forward_and_adjoints(interp_oracle, nside, nside_beam, cap
 
 def test_batched_launch_refuses_interpolation():
     import torch
-    from pisco4cmb_amd import torch_ops
-    from pisco4cmb_amd._lib import check, lib
     from pisco4cmb_amd.batched import DetectorBatch
     ns = 32
     beam = inputs.gaussian_polbeam(256, 4.0, 1.5, nside_sky=ns)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
     batch = DetectorBatch([(t(np.stack(beam.aBeams)), t(np.stack(beam.bBeams)), 256, 0.0)])
-    ctx = torch_ops._ctx(torch.device("cuda:0"))
-    check(lib.pisco_set_tuning(ctx, b"beam_interpolation", 1))
+    batch.set_tuning("beam_interpolation", 1)
     try:
         sky = [torch.zeros(12 * ns * ns, device="cuda") for _ in range(3)]
         ra, dec, pa = (t(x) for x in inputs.raster_scan(10))
         with pytest.raises(PiscoError):
             batch.forward(*sky, ra, dec, pa, ns)
     finally:
-        check(lib.pisco_set_tuning(ctx, b"beam_interpolation", 0))
         batch.close()

@@ -8,7 +8,8 @@ size, plus samples whose discs hold the north pole.  Tolerances: TOD within
 relative RMS 1e-6 of the oracle; sky adjoints within 1e-6 of the oracle's
 FP64 adjoint; a sample's TOD does not depend on the other samples of the
 launch, so launches split at 2^20 samples give the same bits as a launch of
-the tail alone."""
+the tail alone.  Every test runs on a co-polar Gaussian beam and on a
+cross-polar beam (all beam maps non-zero)."""
 import math
 
 import numpy as np
@@ -27,10 +28,10 @@ def rel_rms(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
 
 
-@pytest.fixture(scope="module")
-def case():
+@pytest.fixture(scope="module", params=["copolar", "crosspolar"])
+def case(request):
     sky = inputs.iid_sky(NS, seed=61)
-    beam = inputs.gaussian_polbeam(NB, 1.5, 0.5, 0.05, 0.01, nside_sky=NS)
+    beam = inputs.make_polbeam(request.param, NB, 1.5, 0.5, 0.05, 0.01, nside_sky=NS)
     ra, dec, pa = inputs.ces_scan(2989, t0=100.0)
     # 12 samples circling the north pole at 0.4 deg: discs that hold it
     k = np.arange(12)
