@@ -118,7 +118,8 @@ int pisco_exec_adjoint_map(pisco_ctx *ctx, const float *g_a, const float *g_b,
 
 /* Adjoint with respect to the beam maps (no counterpart in the reference):
  * ga[k], gb[k] receive d<g,TOD>/d aBeams[k], bBeams[k] (npix floats each;
- * k = 3 is zero because the forward does not read V).  Entries may be NULL. */
+ * k = 3 is zero because the forward does not read V, unless the v_term
+ * tuning key is set).  Entries may be NULL. */
 int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
                             float *const ga[4], float *const gb[4]);
 
@@ -149,7 +150,12 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * "adjbeam_lanes" (banded beam adjoint: 8, the default, lanes per sample
  * with 8 samples per wave, or 64),
  * "adjbeam_chunk" (samples per banded work item, default 1024),
- * "adjbeam_rows" (rows per band; 0, the default, as many as fit the LDS);
+ * "adjbeam_rows" (rows per band; 0, the default, as many as fit the LDS),
+ * "v_term" (host-buffer calls: 1 adds the reference GPU kernel's V A3 and
+ * V B3 (cudaConv2.cu:192,204,214,217-218) to pisco_exec_convolution's TOD,
+ * with update_sky's V (zero if it was NULL), and fills the a[3]/b[3]
+ * gradients of pisco_exec_adjoint_beam; 0, the default, follows the CPU
+ * Convolver, which reads no V; changes results);
  * pisco_get_tuning also reads "cu_count". */
 int pisco_set_tuning(pisco_ctx *ctx, const char *key, long value);
 int pisco_get_tuning(pisco_ctx *ctx, const char *key, long *value);
@@ -168,6 +174,16 @@ int pisco_dev_forward(pisco_ctx *ctx, void *stream, int nside_sky,
                       int nside_beam, long npix_beam, const float *beam8,
                       long nsamples, const float *ra, const float *dec,
                       const float *pa, float *tod_a, float *tod_b);
+
+/* Forward with the V terms of the reference's GPU kernel: d_a gains V A3
+ * and d_b V B3 (cudaConv2.cu:192,204,214,217-218), appended last to the
+ * per-visit float expression of convolver.cpp:157-164; V (device,
+ * 12*nside_sky^2 floats) is required.  The beam rows' A3 and B3 columns are
+ * read.  Not available with beam interpolation (PISCO_ERR_INVALID). */
+int pisco_dev_forward_v(pisco_ctx *ctx, void *stream, int nside_sky, const float *I,
+                        const float *Q, const float *U, const float *V, int nside_beam,
+                        long npix_beam, const float *beam8, long nsamples, const float *ra,
+                        const float *dec, const float *pa, float *tod_a, float *tod_b);
 
 /* Adjoint w.r.t. the sky: gI,gQ,gU (device, 12*nside_sky^2 floats each). */
 int pisco_dev_adjoint_map(pisco_ctx *ctx, void *stream, int nside_sky,

@@ -22,8 +22,9 @@
  *     (the device port src/GpuConvolver/cuda/cudaHealpix.cu:30-259 follows the
  *     same healpix_base.cc routines).
  * plus the adjoints the build adds (TOD -> I,Q,U map; TOD -> beam maps), in
- * double precision, and a full-FP64 forward used to state the error of each
- * result against an FP64 oracle.
+ * double precision, a full-FP64 forward used to state the error of each
+ * result against an FP64 oracle, and the V-term mode of the reference's GPU
+ * kernel (V A3, V B3; cudaConv2.cu:204,214,217-218), off by default.
  *
  * Transcendentals: by default sin/cos/acos/atan2 come from pisco_libm.h, the
  * FP64 library the GPU kernels use, so that pixel indices are comparable bit
@@ -680,8 +681,10 @@ void or_polbeam_build(long npix, int nside_beam, int nside_sky, double epsilon,
 typedef struct {
     hpx_t hs, hb;
     const float *I, *Q, *U;
+    const float *V;   /* V-term mode: the sky V map (NULL: off) */
     long npix_beam;
     const float *a[3], *b[3];
+    const float *a3, *b3;   /* aBeams[3], bBeams[3] (V-term mode) */
     double rho_max;
     int interp;   /* beam interpolated from the get_interpol neighbours */
 } conv_t;
@@ -690,6 +693,16 @@ typedef struct {
 static int g_interp = 0;
 
 void or_set_beam_interpolation(int on) { g_interp = on != 0; }
+
+/* V-term mode of the forward (or_set_v_term): the reference's GPU kernel
+ * adds V A3 to d_a and V B3 to d_b (cudaConv2.cu:204,214,217-218), which the
+ * CPU Convolver does not (convolver.cpp:157-164).  With a non-NULL sky V map
+ * the forwards append sV * A3 (sV * B3) as the last term of the per-visit
+ * float expression (FP64 in the FP64 forward); NULL restores the CPU
+ * Convolver.  Process-wide; the nearest-pixel mode only. */
+static const float *g_skyV = 0;
+
+void or_set_v_term(const float *V) { g_skyV = V; }
 
 /* The beam at (rho, sigma) interpolated from its get_interpol neighbours
  * inside the beam map, normalised by their weight sum: the interpolating
@@ -766,7 +779,16 @@ static void o_sample(const conv_t *cv, double ra_bc, double dec_bc,
                 float ba0 = va[0], ba1 = va[1], ba2 = va[2];
                 float bb0 = vb[0], bb1 = vb[1], bb2 = vb[2];
                 float sI = cv->I[skyPix], sQ = cv->Q[skyPix], sU = cv->U[skyPix];
-                if (kind == V_FWD) {
+                if (kind == V_FWD && cv->V) {
+                    /* the same with the V terms appended (cudaConv2.cu:217-218) */
+                    float sV = cv->V[skyPix], ba3 = cv->a3[bpix], bb3 = cv->b3[bpix];
+                    float ta = sI * (ba0) + sQ * (ba1 * c2chi - ba2 * s2chi) +
+                               sU * (ba2 * c2chi + ba1 * s2chi) + sV * ba3;
+                    float tb = sI * (bb0) + sQ * (-bb1 * c2chi + bb2 * s2chi) +
+                               sU * (-bb2 * c2chi - bb1 * s2chi) + sV * bb3;
+                    data_a += ta;
+                    data_b += tb;
+                } else if (kind == V_FWD) {
                     /* float expression, double accumulation (convolver.cpp:157-164) */
                     float ta = sI * (ba0) + sQ * (ba1 * c2chi - ba2 * s2chi) +
                                sU * (ba2 * c2chi + ba1 * s2chi);
@@ -779,6 +801,10 @@ static void o_sample(const conv_t *cv, double ra_bc, double dec_bc,
                               (double)sU * (ba2 * c2d + ba1 * s2d);
                     data_b += (double)sI * bb0 + (double)sQ * (-bb1 * c2d + bb2 * s2d) +
                               (double)sU * (-bb2 * c2d - bb1 * s2d);
+                    if (cv->V) {
+                        data_a += (double)cv->V[skyPix] * cv->a3[bpix];
+                        data_b += (double)cv->V[skyPix] * cv->b3[bpix];
+                    }
                 } else if (kind == V_ADJ_MAP) {
                     double c = c2chi, s = s2chi;
                     double vI = ga * ba0 + gb * bb0;
@@ -838,6 +864,9 @@ static conv_t o_conv_make(int nside_sky, const float *I, const float *Q,
         cv.a[k] = beams[k];
         cv.b[k] = beams[4 + k];
     }
+    cv.a3 = beams[3];
+    cv.b3 = beams[7];
+    cv.V = (g_skyV && beams[3] && beams[7] && !g_interp) ? g_skyV : 0;
     cv.rho_max = or_beam_rho_max(nside_beam, npix_beam);
     cv.interp = g_interp;
     return cv;

@@ -151,6 +151,8 @@ struct pisco_ctx {
     bool has_sky = false, has_beam = false, has_scan = false;
     int nside_sky = 0;
     DevBuf<float> I, Q, U;
+    DevBuf<float> V;              /* sky V when update_sky had one (has_v) */
+    bool has_v = false;
     int nside_beam = 0;
     long npix_beam = 0;
     double rho_max = 0.0;
@@ -181,6 +183,7 @@ struct pisco_ctx {
     int adjbeam_chunk = 1024;     /* samples per work item of the banded beam adjoint */
     int adjbeam_rows = 0;         /* beam rows per band; 0 = as many as fit the LDS */
     int adjbeam_lanes = 8;        /* lanes per sample of the banded beam adjoint: 8 or 64 */
+    int v_term = 0;               /* 1: host-API TOD adds V A3, V B3 as the reference's GPU kernel */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
@@ -189,7 +192,7 @@ struct pisco_ctx {
 
     template <typename F> void each_buf(F f)
     {
-        f(I); f(Q); f(U); f(beam); f(ra); f(dec); f(pa); f(tod); f(cot); f(acc); f(fout);
+        f(I); f(Q); f(U); f(V); f(beam); f(ra); f(dec); f(pa); f(tod); f(cot); f(acc); f(fout);
         f(fxwork); f(fxscale); f(support); f(sky4);
     }
 };
@@ -275,19 +278,25 @@ int adj_lanes(const pisco_ctx *c, const PairParams &p)
 
 /* Launches op over samples p.s_begin .. p.s_end in chunks of ptgPerConv (or
  * one launch). */
-int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st, bool sky_packed = false)
+int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st, bool sky_packed = false,
+             const float *V = nullptr)
 {
-    if (op == OP_FWD && !c->beam_interp && sky_packed) {
+    const bool fwd = op == OP_FWD || op == OP_FWD_V;
+    if (op == OP_FWD_V && c->beam_interp)
+        return fail(PISCO_ERR_INVALID, "the V term is not available with beam interpolation");
+    if (fwd && !c->beam_interp && sky_packed) {
         p.sky4 = (const float4 *)c->sky4.p;   /* packed by the caller on this stream */
-    } else if (op == OP_FWD && !c->beam_interp) {
+    } else if (fwd && !c->beam_interp) {
         /* the nearest-pixel forward reads the sky packed as one float4 per
-         * pixel: one load per visit instead of three */
+         * pixel (I, Q, U and, with the V term, V): one load per visit instead
+         * of three */
         const long npix = p.hs.npix;
         CHK(c->sky4.ensure(4 * (size_t)npix));
-        HIPCHK(launch_pack_sky(p.I, p.Q, p.U, npix, (float4 *)c->sky4.p, st));
+        HIPCHK(launch_pack_sky(p.I, p.Q, p.U, npix, (float4 *)c->sky4.p, st,
+                               op == OP_FWD_V ? V : nullptr));
         p.sky4 = (const float4 *)c->sky4.p;
     }
-    const int lanes = (op == OP_FWD && !c->beam_interp) ? fwd_lanes(c, p) : 64;
+    const int lanes = (fwd && !c->beam_interp) ? fwd_lanes(c, p) : 64;
     const long n = p.s_end, s0 = p.s_begin;
     long chunk = (c->cfg.ptgPerConv > 0) ? c->cfg.ptgPerConv : n - s0;
     if (chunk < (1L << 20))
@@ -298,7 +307,7 @@ int run_pair(pisco_ctx *c, int op, PairParams p, hipStream_t st, bool sky_packed
         if (c->beam_interp)
             HIPCHK(launch_pair_interp(op, p, st));
         else if (lanes < 64)
-            HIPCHK(launch_fwd_sub(lanes, p, st));
+            HIPCHK(launch_fwd_sub(lanes, p, st, op == OP_FWD_V));
         else
             HIPCHK(launch_pair(op, p, st));
     }
@@ -522,6 +531,10 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value < 0 || value > 65536)
             return fail(PISCO_ERR_INVALID, "adjmap_blocks out of range");
         c->adjmap_blocks = (int)value;
+    } else if (k == "v_term") {
+        if (value != 0 && value != 1)
+            return fail(PISCO_ERR_INVALID, "v_term must be 0 or 1");
+        c->v_term = (int)value;
     } else {
         return fail(PISCO_ERR_INVALID, "unknown tuning key '%s'", key);
     }
@@ -569,6 +582,8 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->batch_adj_group;
     else if (k == "cu_count")
         *value = c->ncu;
+    else if (k == "v_term")
+        *value = c->v_term;
     else
         return fail(PISCO_ERR_INVALID, "unknown tuning key '%s'", key);
     return PISCO_OK;
@@ -736,7 +751,8 @@ int pisco_freeze_buffers(pisco_ctx *c, int frozen)
 int pisco_update_sky(pisco_ctx *c, int nside, const float *I, const float *Q,
                      const float *U, const float *V)
 {
-    (void)V;   /* the CPU Convolver does not read V (convolver.cpp:157-164) */
+    /* V is kept for the V-term option (v_term); the CPU Convolver does not
+     * read it (convolver.cpp:157-164) */
     if (!c || !I || !Q || !U)
         return fail(PISCO_ERR_INVALID, "NULL argument");
     if (!valid_nside(nside))
@@ -750,6 +766,11 @@ int pisco_update_sky(pisco_ctx *c, int nside, const float *I, const float *Q,
     HIPCHK(hipMemcpyAsync(c->I.p, I, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->Q.p, Q, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->U.p, U, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    c->has_v = V != nullptr;
+    if (V) {
+        CHK(c->V.ensure(n));
+        HIPCHK(hipMemcpyAsync(c->V.p, V, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
     const RingRec *rt;
     CHK(ring_table(c, nside, &rt));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -854,7 +875,8 @@ int pisco_exec_convolution(pisco_ctx *c, char polflag, float *tod_a, float *tod_
         q.s_end = std::min(n, (k + 1) * part);
         if (q.s_begin >= q.s_end)
             break;
-        rc = run_pair(c, OP_FWD, q, c->stream, k > 0);
+        rc = run_pair(c, c->v_term ? OP_FWD_V : OP_FWD, q, c->stream, k > 0,
+                      c->has_v ? c->V.p : nullptr);
         if (rc == PISCO_OK) {
             hipError_t e = hipEventCreateWithFlags(&ev[nev], hipEventDisableTiming);
             if (e == hipSuccess)
@@ -933,15 +955,23 @@ int pisco_exec_adjoint_beam(pisco_ctx *c, const float *g_a, const float *g_b,
         if (g_b)
             HIPCHK(hipMemcpyAsync(c->cot.p + n, g_b, n * 4, hipMemcpyHostToDevice, c->stream));
     }
-    CHK(c->fout.ensure(8 * (size_t)nb));
+    /* with the V term, A3 and B3 receive g V: the beam adjoint's A0 and B0
+     * columns (g I) of a second pass with V in place of I */
+    const bool vpass = c->v_term && c->has_v;
+    CHK(c->fout.ensure((vpass ? 16 : 8) * (size_t)nb));
     CHK(pisco_dev_adjoint_beam(c, c->stream, c->nside_sky, c->I.p, c->Q.p, c->U.p,
                                c->nside_beam, nb, n, c->ra.p, c->dec.p, c->pa.p,
                                g_a ? c->cot.p : nullptr, g_b ? c->cot.p + n : nullptr,
                                c->fout.p));
+    if (vpass)
+        CHK(pisco_dev_adjoint_beam(c, c->stream, c->nside_sky, c->V.p, c->V.p, c->V.p,
+                                   c->nside_beam, nb, n, c->ra.p, c->dec.p, c->pa.p,
+                                   g_a ? c->cot.p : nullptr, g_b ? c->cot.p + n : nullptr,
+                                   c->fout.p + 8 * nb));
     /* [nb][8] -> 8 maps on the host */
     float *h = nullptr;
-    HIPCHK(hipHostMalloc((void **)&h, sizeof(float) * 8 * (size_t)nb, 0));
-    hipError_t e = hipMemcpyAsync(h, c->fout.p, sizeof(float) * 8 * (size_t)nb,
+    HIPCHK(hipHostMalloc((void **)&h, sizeof(float) * (vpass ? 16 : 8) * (size_t)nb, 0));
+    hipError_t e = hipMemcpyAsync(h, c->fout.p, sizeof(float) * (vpass ? 16 : 8) * (size_t)nb,
                                   hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->stream);
@@ -952,6 +982,13 @@ int pisco_exec_adjoint_beam(pisco_ctx *c, const float *g_a, const float *g_b,
                 if (ga[k]) ga[k][i] = h[8 * i + ia[k]];
                 if (gb[k]) gb[k][i] = h[8 * i + ib[k]];
             }
+        if (vpass) {
+            const float *hv = h + 8 * nb;
+            for (long i = 0; i < nb; i++) {
+                if (ga[3]) ga[3][i] = hv[8 * i + 0];
+                if (gb[3]) gb[3][i] = hv[8 * i + 3];
+            }
+        }
     }
     (void)hipHostFree(h);
     HIPCHK(e);
@@ -989,6 +1026,22 @@ int pisco_dev_forward(pisco_ctx *c, void *stream, int nside_sky, const float *I,
     p.tod_a = tod_a;
     p.tod_b = tod_b;
     return run_pair(c, OP_FWD, p, pick(c, stream));
+}
+
+int pisco_dev_forward_v(pisco_ctx *c, void *stream, int nside_sky, const float *I,
+                        const float *Q, const float *U, const float *V, int nside_beam,
+                        long npix_beam, const float *beam8, long nsamples, const float *ra,
+                        const float *dec, const float *pa, float *tod_a, float *tod_b)
+{
+    if (!c || !I || !Q || !U || !V || !beam8 || (nsamples > 0 && (!ra || !dec || !pa)))
+        return fail(PISCO_ERR_INVALID, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    PairParams p;
+    CHK(setup_pair(c, nside_sky, I, Q, U, nside_beam, npix_beam, beam8, nsamples, ra, dec,
+                   pa, p));
+    p.tod_a = tod_a;
+    p.tod_b = tod_b;
+    return run_pair(c, OP_FWD_V, p, pick(c, stream), false, V);
 }
 
 int pisco_dev_adjoint_map(pisco_ctx *c, void *stream, int nside_sky, int nside_beam,

@@ -140,9 +140,31 @@ struct Pending {
     float c2, s2;
     float4 v0;        /* A0 A1 A2 B0 */
     float2 v1;        /* B1 B2 */
+    float2 v2;        /* A3 B3 (forward with the V term only) */
     float sI, sQ, sU; /* sky of the visit (forward only) */
+    float sV;         /* sky V of the visit (forward with the V term only) */
     bool live;
 };
+
+/* The TOD terms of one visit, a and b: the float expression of
+ * convolver.cpp:157-164 and, with VT, the V terms the reference's GPU kernel
+ * adds (V A3, V B3; cudaConv2.cu:204,214,217-218), appended last. */
+template <bool VT>
+__device__ inline void visit_tod(const Pending &e, float &ta, float &tb)
+{
+    const float A0 = e.v0.x, A1 = e.v0.y, A2 = e.v0.z;
+    const float B0 = e.v0.w, B1 = e.v1.x, B2 = e.v1.y;
+    const float c2 = e.c2, s2 = e.s2;
+    if constexpr (VT) {
+        ta = e.sI * (A0) + e.sQ * (A1 * c2 - A2 * s2) + e.sU * (A2 * c2 + A1 * s2) +
+             e.sV * e.v2.x;
+        tb = e.sI * (B0) + e.sQ * (-B1 * c2 + B2 * s2) + e.sU * (-B2 * c2 - B1 * s2) +
+             e.sV * e.v2.y;
+    } else {
+        ta = e.sI * (A0) + e.sQ * (A1 * c2 - A2 * s2) + e.sU * (A2 * c2 + A1 * s2);
+        tb = e.sI * (B0) + e.sQ * (-B1 * c2 + B2 * s2) + e.sU * (-B2 * c2 - B1 * s2);
+    }
+}
 
 __device__ inline int lanes_below(unsigned long long m)
 {
@@ -157,7 +179,7 @@ __device__ inline int lanes_below(unsigned long long m)
  * active.  The double-precision form costs about a quarter of the exact
  * path: draining through it measured 2% faster on the 5-degree, NSIDE 512
  * sky adjoint. */
-template <bool LOAD_SKY, typename Use>
+template <bool LOAD_SKY, typename Use, bool VT = false>
 __device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, int cnt, int lane,
                                    const Centre &c, double c2psi, double s2psi, Use &use)
 {
@@ -179,11 +201,14 @@ __device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, i
             const float *row = p.beam + 8 * (long)e.bp;
             q.v0 = *(const float4 *)row;
             q.v1 = *(const float2 *)(row + 4);
+            if constexpr (VT)
+                q.v2 = *(const float2 *)(row + 6);
             if (LOAD_SKY) {
                 const float4 sv = p.sky4[pix];
                 q.sI = sv.x;
                 q.sQ = sv.y;
                 q.sU = sv.z;
+                q.sV = sv.w;
             }
             q.live = true;
             use(q);
@@ -213,7 +238,7 @@ __device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, i
  * mapping cannot certify go to the queue (qn entries, wave-uniform); a full
  * batch of 64 is drained at once.  use(q) is called once per visit whose
  * beam pixel lies inside the beam. */
-template <bool LOAD_SKY, bool MAP64, typename Use>
+template <bool LOAD_SKY, bool MAP64, typename Use, bool VT = false>
 __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, int lane,
                                    const Centre &c, const SampleF &sf, const SampleD &sd,
                                    int &qn, Use &use)
@@ -280,7 +305,8 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
              * loads are not yet issued: little is live across the exact
              * path's out-of-line call */
             if (qn >= kWaveSize) {
-                drain_exact<LOAD_SKY>(p, t, qn - kWaveSize, kWaveSize, lane, c, sd.c2psi, sd.s2psi, use);
+                drain_exact<LOAD_SKY, Use, VT>(p, t, qn - kWaveSize, kWaveSize, lane, c, sd.c2psi,
+                                               sd.s2psi, use);
                 qn -= kWaveSize;
             }
         }
@@ -297,11 +323,14 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
             const float *row = p.beam + 8 * (long)(q.live ? bp : 0);
             q.v0 = *(const float4 *)row;
             q.v1 = *(const float2 *)(row + 4);
+            if constexpr (VT)
+                q.v2 = *(const float2 *)(row + 6);
             const int sp = q.live ? pix : 0;
             const float4 sv = p.sky4[sp];
             q.sI = sv.x;
             q.sQ = sv.y;
             q.sU = sv.z;
+            q.sV = sv.w;
         } else if (q.live) {
             /* adjoints: only live visits load (measured faster there) */
             const float *row = p.beam + 8 * (long)bp;
@@ -349,13 +378,14 @@ template <int OP>
 __device__ inline void pair_sample(const PairParams &p, long i, float pa_i, RunTable &t,
                                    int lane)
 {
+    constexpr bool FWD = OP == OP_FWD || OP == OP_FWD_V;
     Centre c;
     SampleF sf;
     SampleD sd;
     Disc d;
     sample_setup(p, i, pa_i, c, sf, sd, d);
     double ga = 0.0, gb = 0.0;
-    if (OP != OP_FWD) {
+    if (!FWD) {
         ga = uni(p.g_a ? (double)p.g_a[i] : 0.0);
         gb = uni(p.g_b ? (double)p.g_b[i] : 0.0);
     }
@@ -365,10 +395,9 @@ __device__ inline void pair_sample(const PairParams &p, long i, float pa_i, RunT
         const float B0 = q.v0.w, B1 = q.v1.x, B2 = q.v1.y;
         const float c2 = q.c2, s2 = q.s2;
         const int pix = q.pix, bp = q.bp;
-        if (OP == OP_FWD) {
-            float sI = q.sI, sQ = q.sQ, sU = q.sU;
-            float ta = sI * (A0) + sQ * (A1 * c2 - A2 * s2) + sU * (A2 * c2 + A1 * s2);
-            float tb = sI * (B0) + sQ * (-B1 * c2 + B2 * s2) + sU * (-B2 * c2 - B1 * s2);
+        if (FWD) {
+            float ta, tb;
+            visit_tod<OP == OP_FWD_V>(q, ta, tb);
             acc_a += ta;
             acc_b += tb;
         } else if (OP == OP_ADJ_MAP) {
@@ -395,12 +424,13 @@ __device__ inline void pair_sample(const PairParams &p, long i, float pa_i, RunT
     int qn = 0;   /* deferred visits (wave-uniform) */
     for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
         int total = build_runs(p.hs, d, r0, lane, t);
-        walk_visits<OP == OP_FWD, OP == OP_FWD>(p, t, total, lane, c, sf, sd, qn, use);
+        walk_visits<FWD, FWD, decltype(use), OP == OP_FWD_V>(p, t, total, lane, c, sf, sd, qn,
+                                                             use);
         wave_sync();
     }
-    if (OP != OP_FWD && qn > 0)   /* the forward's mapping never defers */
+    if (!FWD && qn > 0)   /* the forward's mapping never defers */
         drain_exact<false>(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, use);
-    if (OP == OP_FWD) {
+    if (FWD) {
         acc_a = wave_sum(acc_a);
         acc_b = wave_sum(acc_b);
         if (lane == 0) {
@@ -501,7 +531,7 @@ __device__ inline int build_sub(const Hpx &hs, const Disc &d, int nrings, int ps
  * computes the first visit of its run (or, in a group with no visit, pixel 0
  * of ring 1) and discards it.  use(q) is called once per visit whose beam
  * pixel lies inside the beam. */
-template <int SUB, bool LOAD_SKY, typename Use>
+template <int SUB, bool LOAD_SKY, typename Use, bool VT = false>
 __device__ inline void walk_sub(const PairParams &p, const SubTable<SUB> &t, int total,
                                 int wtot, int sl, const Centre &c, double c2psi, double s2psi,
                                 Use &use)
@@ -545,10 +575,13 @@ __device__ inline void walk_sub(const PairParams &p, const SubTable<SUB> &t, int
             const float *row = p.beam + 8 * (long)(q.live ? bp : 0);
             q.v0 = *(const float4 *)row;
             q.v1 = *(const float2 *)(row + 4);
+            if constexpr (VT)
+                q.v2 = *(const float2 *)(row + 6);
             const float4 sv = p.sky4[q.live ? pix : 0];
             q.sI = sv.x;
             q.sQ = sv.y;
             q.sU = sv.z;
+            q.sV = sv.w;
         } else if (q.live) {
             const float *row = p.beam + 8 * (long)bp;
             q.v0 = *(const float4 *)row;
@@ -571,7 +604,7 @@ __device__ inline int sub_setup(const PairParams &p, long i, bool valid, Centre 
     return valid && d.r_last >= d.r_first ? d.r_last - d.r_first + 1 : 0;
 }
 
-template <int SUB>
+template <int SUB, bool VT = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_fwd_sub(PairParams p)
 {
     constexpr int G = kWaveSize / SUB;
@@ -593,17 +626,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     const int npass = (wave_max_uniform(nrings) + SUB - 1) / SUB;
     double acc_a = 0.0, acc_b = 0.0;
     auto use = [&](const Pending &e) {
-        const float A0 = e.v0.x, A1 = e.v0.y, A2 = e.v0.z;
-        const float B0 = e.v0.w, B1 = e.v1.x, B2 = e.v1.y;
-        const float c2 = e.c2, s2 = e.s2;
-        float ta = e.sI * (A0) + e.sQ * (A1 * c2 - A2 * s2) + e.sU * (A2 * c2 + A1 * s2);
-        float tb = e.sI * (B0) + e.sQ * (-B1 * c2 + B2 * s2) + e.sU * (-B2 * c2 - B1 * s2);
+        float ta, tb;
+        visit_tod<VT>(e, ta, tb);
         acc_a += ta;
         acc_b += tb;
     };
     for (int ps = 0; ps < npass; ps++) {
         const int total = build_sub<SUB>(p.hs, d, nrings, ps, sl, t);
-        walk_sub<SUB, true>(p, t, total, wave_max_uniform(total), sl, c, c2psi, s2psi, use);
+        walk_sub<SUB, true, decltype(use), VT>(p, t, total, wave_max_uniform(total), sl, c, c2psi,
+                                               s2psi, use);
         wave_sync();
     }
 #pragma unroll
@@ -1984,11 +2015,12 @@ __global__ void k_add_offset(const float *x, float off, float *y, long n)
         y[i] = x[i] + off;
 }
 
-__global__ void k_pack_sky(const float *I, const float *Q, const float *U, long n, float4 *out)
+__global__ void k_pack_sky(const float *I, const float *Q, const float *U, const float *V, long n,
+                           float4 *out)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (long)gridDim.x * blockDim.x)
-        out[i] = make_float4(I[i], Q[i], U[i], 0.f);
+        out[i] = make_float4(I[i], Q[i], U[i], V ? V[i] : 0.f);
 }
 
 /* Deterministic sky adjoint: out[0] = max over beam rows of the sum of |A0
@@ -2225,6 +2257,7 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
     unsigned g = nblocks(n, kWavesPerBlock);
     switch (op) {
     case OP_FWD: hipLaunchKernelGGL(k_pair<OP_FWD>, dim3(g), dim3(kBlock), 0, st, p); break;
+    case OP_FWD_V: hipLaunchKernelGGL(k_pair<OP_FWD_V>, dim3(g), dim3(kBlock), 0, st, p); break;
     case OP_ADJ_MAP: hipLaunchKernelGGL(k_pair<OP_ADJ_MAP>, dim3(g), dim3(kBlock), 0, st, p); break;
     case OP_ADJ_BEAM: hipLaunchKernelGGL(k_pair<OP_ADJ_BEAM>, dim3(g), dim3(kBlock), 0, st, p); break;
     default: return hipErrorInvalidValue;
@@ -2232,27 +2265,33 @@ hipError_t launch_pair(int op, const PairParams &p, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st)
+template <bool VT>
+static hipError_t launch_fwd_sub_vt(int sub, const PairParams &p, long n, hipStream_t st)
 {
-    long n = p.s_end - p.s_begin;
-    if (n <= 0)
-        return hipSuccess;
     switch (sub) {
     case 8:
-        hipLaunchKernelGGL(k_fwd_sub<8>, dim3(nblocks(n, kWavesPerBlock * 8)), dim3(kBlock), 0,
-                           st, p);
+        hipLaunchKernelGGL((k_fwd_sub<8, VT>), dim3(nblocks(n, kWavesPerBlock * 8)), dim3(kBlock),
+                           0, st, p);
         break;
     case 16:
-        hipLaunchKernelGGL(k_fwd_sub<16>, dim3(nblocks(n, kWavesPerBlock * 4)), dim3(kBlock), 0,
-                           st, p);
+        hipLaunchKernelGGL((k_fwd_sub<16, VT>), dim3(nblocks(n, kWavesPerBlock * 4)),
+                           dim3(kBlock), 0, st, p);
         break;
     case 32:
-        hipLaunchKernelGGL(k_fwd_sub<32>, dim3(nblocks(n, kWavesPerBlock * 2)), dim3(kBlock), 0,
-                           st, p);
+        hipLaunchKernelGGL((k_fwd_sub<32, VT>), dim3(nblocks(n, kWavesPerBlock * 2)),
+                           dim3(kBlock), 0, st, p);
         break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st, bool vterm)
+{
+    long n = p.s_end - p.s_begin;
+    if (n <= 0)
+        return hipSuccess;
+    return vterm ? launch_fwd_sub_vt<true>(sub, p, n, st) : launch_fwd_sub_vt<false>(sub, p, n, st);
 }
 
 hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st)
@@ -2481,11 +2520,12 @@ hipError_t launch_add_offset(const float *x, float off, float *y, long n, hipStr
 }
 
 hipError_t launch_pack_sky(const float *I, const float *Q, const float *U, long npix,
-                           float4 *sky4, hipStream_t st)
+                           float4 *sky4, hipStream_t st, const float *V)
 {
     if (npix <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_pack_sky, dim3(gs_blocks(npix)), dim3(256), 0, st, I, Q, U, npix, sky4);
+    hipLaunchKernelGGL(k_pack_sky, dim3(gs_blocks(npix)), dim3(256), 0, st, I, Q, U, V, npix,
+                       sky4);
     return hipGetLastError();
 }
 

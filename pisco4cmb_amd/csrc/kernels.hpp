@@ -10,7 +10,9 @@
 
 namespace pisco {
 
-enum { OP_FWD = 0, OP_ADJ_MAP = 1, OP_ADJ_BEAM = 2 };
+/* OP_FWD_V: the forward with the V terms of the reference's GPU kernel
+ * (V A3, V B3, cudaConv2.cu:204,214,217-218); p.sky4 then carries V. */
+enum { OP_FWD = 0, OP_ADJ_MAP = 1, OP_ADJ_BEAM = 2, OP_FWD_V = 3 };
 
 /* One launch over samples [s_begin, s_end) of a detector pair a/b that
  * share pointing and beam geometry. */
@@ -19,7 +21,7 @@ struct PairParams {
     Hpx hs;
     const RingRec *rings;     /* per-ring constants, index 1..4N-1 */
     const float *I, *Q, *U;
-    const float4 *sky4;       /* forward: I, Q, U, 0 per pixel (packed from I, Q, U) */
+    const float4 *sky4;       /* forward: I, Q, U, V (0 without the V term) per pixel */
     /* beam: first npix_beam RING pixels of a map of NSIDE hb.nside, rows of
      * eight floats A0 A1 A2 B0 B1 B2 A3 B3 (what the CPU forward reads comes
      * first: one 16-byte and one 8-byte load per visit) */
@@ -137,9 +139,9 @@ hipError_t launch_batch_forward(int group, const BatchParams &p, hipStream_t st)
 hipError_t launch_pair(int op, const PairParams &p, hipStream_t st);
 /* The beam-interpolation mode of the same operations (k_pair_interp). */
 hipError_t launch_pair_interp(int op, const PairParams &p, hipStream_t st);
-/* Forward with 64/sub samples per wave, sub = 16 or 32 lanes each (small
- * discs); same results as launch_pair(OP_FWD). */
-hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st);
+/* Forward with 64/sub samples per wave, sub = 8, 16 or 32 lanes each (small
+ * discs); same results as launch_pair(OP_FWD), or OP_FWD_V with vterm. */
+hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st, bool vterm = false);
 /* Beam adjoint in LDS bands of beam rows (work items: band x chunk of
  * samples); p.gbeam accumulates in FP64, p.sky4 holds the packed sky.  rows:
  * rows per band (0 or more than fits: as many as fit the LDS). */
@@ -176,9 +178,9 @@ int adjmap_describe(char *buf, size_t n, int cap, int rw);
 hipError_t launch_beam_support(const float *beam8, long npix, int *last, hipStream_t st);
 hipError_t launch_ring_table(const Hpx &h, RingRec *out, hipStream_t st);
 hipError_t launch_zero_f64(double *x, long n, hipStream_t st);
-/* sky4[p] = (I[p], Q[p], U[p], 0) */
+/* sky4[p] = (I[p], Q[p], U[p], V[p]), V[p] = 0 if V is NULL */
 hipError_t launch_pack_sky(const float *I, const float *Q, const float *U, long npix,
-                           float4 *sky4, hipStream_t st);
+                           float4 *sky4, hipStream_t st, const float *V = nullptr);
 hipError_t launch_f64_to_f32(const double *x, float *y, long n, hipStream_t st);
 /* y = x + off (float), n elements */
 hipError_t launch_add_offset(const float *x, float off, float *y, long n, hipStream_t st);

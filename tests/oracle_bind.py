@@ -62,6 +62,7 @@ class Oracle:
             "or_skymap_solve": (c_long, [c_int, P, P, P, P, P, P]),
             "or_dgesv3_many": (None, [c_long, P, P, P]),
             "or_set_beam_interpolation": (None, [c_int]),
+            "or_set_v_term": (None, [P]),
             "or_focal_plane_to_sky": (None, [c_int, P, P, P, c_long, P, P, P, c_int, P, P, P]),
         }
         for k, (r, a) in sig.items():
@@ -257,6 +258,12 @@ class Oracle:
         """Selects the oracle's beam interpolation mode for convolve and the
         adjoints (process-wide; tests restore it)."""
         self.L.or_set_beam_interpolation(int(bool(on)))
+
+    def set_v_term(self, V):
+        """V-term mode of the forward (process-wide; tests restore it with
+        None): the sky V map, or None for the CPU Convolver."""
+        self._v = None if V is None else np.ascontiguousarray(V, np.float32)
+        self.L.or_set_v_term(_a(self._v))
 
     # --- BPoint
     def focal_plane_to_sky(self, dx, dy, pol, x_ac, y_ac, rot_ac, flag):

@@ -218,3 +218,69 @@ def test_identity_catches_planted_mistakes(oracle, xbeam, scan):
                 NS, 0 * I, Q, U, NB, beams, ra, dec, pa, precision64=True)[0]) < 1e-9
         else:
             assert rel(lhs, rhs) > 1e-2, form
+
+
+# ---- V-term mode (the reference GPU kernel's V A3, V B3) --------------------
+
+def test_v_term_is_the_I_term_of_the_V_columns(oracle, xbeam, scan):
+    """With I = Q = U = 0, the V-term TOD is sum V_p A3[b] (B3 for b): bit for
+    bit the CPU Convolver's TOD of the sky I := V with a beam whose A0, B0
+    are A3, B3.  With V = None the mode is off (the CPU Convolver's bits)."""
+    _, _, _, beams = xbeam
+    ra, dec, pa = scan
+    rng = np.random.default_rng(51)
+    npix = 12 * NS * NS
+    V = rng.normal(size=npix).astype(np.float32)
+    z = np.zeros(npix, np.float32)
+    vbeam = [beams[3], beams[1], beams[2], beams[3], beams[7], beams[5], beams[6], beams[7]]
+    ref = oracle.convolve(NS, V, z, z, NB, vbeam, ra, dec, pa)
+    oracle.set_v_term(V)
+    try:
+        got = oracle.convolve(NS, z, z, z, NB, beams, ra, dec, pa)
+    finally:
+        oracle.set_v_term(None)
+    for x, y in zip(got, ref):
+        assert np.array_equal(x, y)
+    I, Q, U, _ = inputs.iid_sky(NS, 31)
+    a0 = oracle.convolve(NS, I, Q, U, NB, beams, ra, dec, pa)
+    a1 = oracle.convolve(NS, I, Q, U, NB, beams, ra, dec, pa)
+    assert all(np.array_equal(x, y) for x, y in zip(a0, a1))
+
+
+def test_v_term_linearity_and_circular_beam(oracle, scan):
+    """FP64: TOD_V(I, Q, U, V) = TOD(I, Q, U) + TOD_V(0, 0, 0, V) (1e-12).
+    A circularly polarised beam (E_cx = i E_co, no leakage) has tilde
+    V = tilde I and tilde Q = tilde U = 0, so A3 = A0, A1 = A2 = 0, and a sky
+    with V = I sees twice its I TOD (1e-12)."""
+    ra, dec, pa = scan
+    npix = pypisco.beam_npix_for_radius(NB, CAP * DEG)
+    f = inputs.crosspolar_fields(NB, npix, FWHM, 0.05)
+    circ = (f[0], f[1], f[0], f[1] + math.pi / 2)
+    beams = oracle.polbeam_from_fields(npix, NB, NS, 0.0, circ, circ)
+    assert np.allclose(beams[3], beams[0], rtol=1e-6, atol=0)
+    assert np.abs(beams[1]).max() <= 1e-6 * beams[0].max()
+    assert np.abs(beams[2]).max() <= 1e-6 * beams[0].max()
+    I, Q, U, _ = inputs.iid_sky(NS, 33)
+    V = np.random.default_rng(52).normal(size=I.size).astype(np.float32)
+    z = np.zeros_like(I)
+    xb = inputs.crosspolar_polbeam(NB, CAP, FWHM, 0.05, 0.01, nside_sky=NS)
+    xbl = list(xb.aBeams) + list(xb.bBeams)
+    base = oracle.convolve(NS, I, Q, U, NB, xbl, ra, dec, pa, precision64=True)
+    oracle.set_v_term(V)
+    try:
+        full = oracle.convolve(NS, I, Q, U, NB, xbl, ra, dec, pa, precision64=True)
+        vonly = oracle.convolve(NS, z, z, z, NB, xbl, ra, dec, pa, precision64=True)
+    finally:
+        oracle.set_v_term(None)
+    for k in range(2):
+        assert rel(full[k], base[k] + vonly[k]) < 1e-12
+        assert np.abs(vonly[k]).max() > 1e-3 * np.abs(base[k]).max()   # not empty
+    # circular beam, V = I
+    twice = oracle.convolve(NS, I, z, z, NB, beams, ra, dec, pa, precision64=True)
+    oracle.set_v_term(I)
+    try:
+        both = oracle.convolve(NS, I, z, z, NB, beams, ra, dec, pa, precision64=True)
+    finally:
+        oracle.set_v_term(None)
+    for k in range(2):
+        assert rel(both[k], 2 * twice[k]) < 1e-6   # A3 = A0 to float rounding
