@@ -175,6 +175,22 @@ int pisco_dev_forward(pisco_ctx *ctx, void *stream, int nside_sky,
                       long nsamples, const float *ra, const float *dec,
                       const float *pa, float *tod_a, float *tod_b);
 
+/* No counterpart in the reference: pisco_dev_forward for callers that
+ * cannot allocate device memory during the call (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It takes no
+ * context and allocates nothing: its workspace (the sky's ring table and
+ * packed pixels) is a fixed device array of 48.1 MiB that each device the
+ * library runs on holds for the life of the process, so the sky NSIDE is at
+ * most 512 (a larger one is refused with PISCO_ERR_INVALID; pisco_dev_forward
+ * takes any).  Runs on the current device and the given stream with the
+ * default launch shapes; results equal pisco_dev_forward's bit for bit.
+ * Every call on a device shares that device's workspace: calls that overlap
+ * in time on one device (other streams, other host threads) race. */
+int pisco_dev_forward_noalloc(void *stream, int nside_sky, const float *I, const float *Q,
+                              const float *U, int nside_beam, long npix_beam,
+                              const float *beam8, long nsamples, const float *ra,
+                              const float *dec, const float *pa, float *tod_a, float *tod_b);
+
 /* Forward with the V terms of the reference's GPU kernel: d_a gains V A3
  * and d_b V B3 (cudaConv2.cu:192,204,214,217-218), appended last to the
  * per-visit float expression of convolver.cpp:157-164; V (device,

@@ -62,6 +62,8 @@ SIGNATURES = {
     "pisco_dev_pack_beam": (c_int, [c_void_p, c_void_p, c_long, P, P]),
     "pisco_dev_forward": (c_int, [c_void_p, c_void_p, c_int, P, P, P, c_int, c_long, P,
                                   c_long, P, P, P, P, P]),
+    "pisco_dev_forward_noalloc": (c_int, [c_void_p, c_int, P, P, P, c_int, c_long, P, c_long, P,
+                                          P, P, P, P]),
     "pisco_dev_forward_v": (c_int, [c_void_p, c_void_p, c_int, P, P, P, P, c_int, c_long, P,
                                     c_long, P, P, P, P, P]),
     "pisco_dev_adjoint_map": (c_int, [c_void_p, c_void_p, c_int, c_int, c_long, P, c_long,

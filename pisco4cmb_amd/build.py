@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpisco.so")
-SOURCES = ["kernels.hip", "skymap.hip", "focalplane.hip", "textio.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "skymap.hip", "focalplane.hip", "textio.hip", "capi.hip",
+           "noalloc.hip"]
 ARCH = os.environ.get("PISCO_OFFLOAD_ARCH", "gfx950")
 
 
