@@ -394,14 +394,17 @@ int pisco_dev_match(pisco_ctx *ctx, void *stream, int nside_sky,
                     int nside_beam, long n, const float *ra, const float *dec,
                     const float *pa, const long *offsets, const int *pixels,
                     int *bpix, double *chi);
-/* As pisco_dev_match (mode 0).  Mode 1 evaluates what the convolution
- * kernels use -- the certified single-precision form, with the exact FP64
- * mapping wherever that form cannot certify a pixel (same beam pixels by
+/* As pisco_dev_match (mode 0).  Mode 1 evaluates what the adjoint kernels
+ * use -- the certified single-precision form, with the exact FP64 mapping
+ * wherever that form cannot certify a pixel (same beam pixels by
  * construction) -- and writes float cos(2chi), sin(2chi) packed into each
  * 8-byte chi element instead of chi.  Mode 2 is the single-precision form
  * alone: bpix = -2 where it defers to the exact mapping.  Mode 3 is the
- * certified double-precision form the forward kernel uses (same output
- * layout as mode 1). */
+ * certified double-precision form of the small-disc kernels (same output
+ * layout as mode 1).  Mode 4 is what the one-wave-per-sample forward uses:
+ * the double-precision form with a float atan2 for the position angle,
+ * and mode 3 where that cannot certify a pixel; mode 5 is that form alone
+ * (bpix = -2 where it declines). */
 int pisco_dev_match_mode(pisco_ctx *ctx, void *stream, int mode, int nside_sky,
                          int nside_beam, long n, const float *ra,
                          const float *dec, const float *pa, const long *offsets,

@@ -1217,8 +1217,8 @@ int pisco_dev_match_mode(pisco_ctx *c, void *stream, int mode, int nside_sky, in
     HIPCHK(hipSetDevice(c->device));
     const RingRec *rt;
     CHK(ring_table(c, nside_sky, &rt));
-    if (mode < 0 || mode > 3)
-        return fail(PISCO_ERR_INVALID, "match mode must be 0, 1, 2 or 3");
+    if (mode < 0 || mode > 5)
+        return fail(PISCO_ERR_INVALID, "match mode must be 0 to 5");
     HIPCHK(launch_match(hpx_make(nside_sky), rt, hpx_make(nside_beam), mode, n, ra, dec, pa,
                         offsets, pixels, bpix, chi, pick(c, stream)));
     return PISCO_OK;

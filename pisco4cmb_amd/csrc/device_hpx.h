@@ -888,6 +888,65 @@ __device__ inline double recip_fast(double n)
     return fma(r, fma(-n, r, 1.0), r);
 }
 
+/* Bound on |alpha_f - alpha| for alpha_f = plf_atan2_core of the FP64
+ * arguments rounded to float (float32 emulation over 4e6 arguments of every
+ * direction and magnitude, the reciprocal estimate within +-1 ulp: at most
+ * 3.6e-7 rad), with a factor of four of margin. */
+constexpr double kAlphaF32Err = 1.5e-6;
+
+/* sigma -> the polar-branch floors of loc2pix (beam_pixel_f64's tail).
+ * et bounds the error of tt = sigma 2/pi (mod 4) coming from an approximate
+ * alpha (0 for the exact alpha): every floor that depends on tt must then be
+ * farther than its error from an integer.  An uncertain floor gives
+ * kNeedExact when DEFER, else the exact path's pixel. */
+template <bool DEFER>
+__device__ inline int beam_floors(const Hpx &hb, double crho, double sigma, double tmp, double y,
+                                  double marg_k, double et)
+{
+    double tt = fmodulo(sigma * kInvHalfPi, 4.0);
+    double tp = tt - (int)tt;
+    double u1 = tp * tmp;
+    double u2 = (1.0 - tp) * tmp;
+    int jp = (int)u1, jm = (int)u2;
+    double d1 = u1 - jp, d2 = u2 - jm;
+    /* margin: nside * 4.4e-14 / sqrt(omc) + 3e-13 tmp + 1e-10, with
+     * 1/sqrt(omc) = sqrt(3) * y; plus et * tmp for the floors of tp * tmp */
+    double md = fma(marg_k, y, fma(fma(et, 1.0, 3e-13), tmp, 1e-10));
+    bool ok = (crho > kTwoThird + 1e-12) && (d1 > md) && (d1 < 1.0 - md) && (d2 > md) &&
+              (d2 < 1.0 - md);
+    int ir = jp + jm + 1;
+    double vv = tt * ir;
+    int ip = (int)vv;
+    if (et > 0.0) {
+        /* the quadrant (int)tt and ip = (int)(tt ir) */
+        double mv = fma(et, (double)ir, 1e-12);
+        double dv = vv - ip;
+        ok = ok && (tp > et) && (tp < 1.0 - et) && (dv > mv) && (dv < 1.0 - mv);
+    }
+    if (ok)
+        return 2 * ir * (ir - 1) + ip;
+    if constexpr (DEFER)
+        return kNeedExact;
+    else
+        return beam_pixel_exact_rho(hb.nside, crho, sigma);
+}
+
+__device__ inline double sigma_of(double alpha, double psi)
+{
+    double sigma = kPi - alpha - psi;
+    if (sigma < 0.0)
+        sigma += 2.0 * kPi;
+    else if (sigma > 2.0 * kPi)
+        sigma -= 2.0 * kPi;
+    return sigma;
+}
+
+/* FLOAT_ALPHA: alpha from the float atan2 of the FP64 arguments, its floors
+ * certified against kAlphaF32Err, and kNeedExact for a visit whose floors
+ * cannot be certified (the caller defers it to this function's exact form);
+ * otherwise alpha from the FP64 atan2, the CPU path's own (sphtrigo.cpp:42),
+ * and the exact path in place for an uncertain floor. */
+template <bool FLOAT_ALPHA = false>
 __device__ inline int beam_pixel_f64(const Hpx &hb, const FastConsts64 &fc, const Centre &c,
                                       double c2psi, double s2psi, double ra_pix,
                                       double cdp, double sdp, float &c2, float &s2)
@@ -904,12 +963,6 @@ __device__ inline int beam_pixel_f64(const Hpx &hb, const FastConsts64 &fc, cons
         big = rho_above_1e6(crho);
     double yb = sd * c.cdc, xb = c.sdc * cdp - c.cdc * sdp * cd;
     double ya = sd * cdp, xa = sdp * c.cdc - cdp * c.sdc * cd;
-    double alpha = big ? pl_atan2(ya, xa) : kHalfPi;
-    double sigma = kPi - alpha - c.psi;
-    if (sigma < 0.0)
-        sigma += 2.0 * kPi;
-    else if (sigma > 2.0 * kPi)
-        sigma -= 2.0 * kPi;
 
     /* cos/sin 2chi */
     double er = 1.0, ei = 0.0;   /* e^{2i(alpha+beta)}; alpha = beta = pi/2 -> 1 */
@@ -921,7 +974,7 @@ __device__ inline int beam_pixel_f64(const Hpx &hb, const FastConsts64 &fc, cons
             er = (wr * wr - wi * wi) * inv;
             ei = 2.0 * wr * wi * inv;
         } else {   /* degenerate geometry: take the angles themselves */
-            double2 e = e2i_angles(yb, xb, alpha);
+            double2 e = e2i_angles(yb, xb, pl_atan2(ya, xa));
             er = e.x;
             ei = e.y;
         }
@@ -931,7 +984,7 @@ __device__ inline int beam_pixel_f64(const Hpx &hb, const FastConsts64 &fc, cons
 
     /* beam pixel: polar branch of loc2pix with the certified floors.
      * sqrt(3 omc) from the hardware rsqrt estimate and two Newton steps:
-     * within 1e-15 relative, an error the margin below includes. */
+     * within 1e-15 relative, an error the margin includes. */
     double omc = 1.0 - crho;
     omc = omc > 0.0 ? omc : 0.0;
     double x3 = 3.0 * omc;
@@ -939,23 +992,19 @@ __device__ inline int beam_pixel_f64(const Hpx &hb, const FastConsts64 &fc, cons
     y = y * fma(-0.5 * x3, y * y, 1.5);
     y = y * fma(-0.5 * x3, y * y, 1.5);
     double tmp = (x3 > 0.0) ? hb.nside * (x3 * y) : 0.0;
-    double tt = fmodulo(sigma * kInvHalfPi, 4.0);
-    double tp = tt - (int)tt;
-    double u1 = tp * tmp;
-    double u2 = (1.0 - tp) * tmp;
-    int jp = (int)u1, jm = (int)u2;
-    double d1 = u1 - jp, d2 = u2 - jm;
-    /* margin: nside * 4.4e-14 / sqrt(omc) + 3e-13 tmp + 1e-10, with
-     * 1/sqrt(omc) = sqrt(3) * y */
-    double md = fma(fc.marg_k, y, fma(3e-13, tmp, 1e-10));
-    bool ok = (crho > kTwoThird + 1e-12) && (d1 > md) && (d1 < 1.0 - md) && (d2 > md) &&
-              (d2 < 1.0 - md);
-    if (ok) {
-        int ir = jp + jm + 1;
-        int ip = (int)(tt * ir);
-        return 2 * ir * (ir - 1) + ip;
+    if constexpr (FLOAT_ALPHA) {
+        double alpha = kHalfPi;
+        if (big) {
+            const float yf = (float)ya, xf = (float)xa;
+            alpha = (double)plf_atan2_core(
+                yf, xf, __builtin_amdgcn_rcpf(fmaxf(fabsf(xf), fabsf(yf))));
+        }
+        return beam_floors<true>(hb, crho, sigma_of(alpha, c.psi), tmp, y, fc.marg_k,
+                                 big ? kAlphaF32Err * kInvHalfPi : 0.0);
+    } else {
+        double alpha = big ? pl_atan2(ya, xa) : kHalfPi;
+        return beam_floors<false>(hb, crho, sigma_of(alpha, c.psi), tmp, y, fc.marg_k, 0.0);
     }
-    return beam_pixel_exact_rho(hb.nside, crho, sigma);
 }
 
 /* get_interpol (Healpix_Base::get_interpol; the device port in the reference

@@ -234,10 +234,11 @@ __device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, i
  * discard it): a load skipped on some path makes the in-order load counter
  * ambiguous at the loop head, and the compiler then waits for all loads
  * there instead of only for the ring record (the adjoints, with fewer loads
- * per visit, measured faster with conditional loads).  Visits the float
- * mapping cannot certify go to the queue (qn entries, wave-uniform); a full
- * batch of 64 is drained at once.  use(q) is called once per visit whose
- * beam pixel lies inside the beam. */
+ * per visit, measured faster with conditional loads).  Visits whose floors
+ * the fast form cannot certify (the float mapping, or the FP64 form with its
+ * float alpha) go to the queue (qn entries, wave-uniform); a full batch of 64
+ * is drained at once.  use(q) is called once per visit whose beam pixel lies
+ * inside the beam. */
 template <bool LOAD_SKY, bool MAP64, typename Use, bool VT = false>
 __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, int lane,
                                    const Centre &c, const SampleF &sf, const SampleD &sd,
@@ -272,8 +273,8 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
                     s++;
             rr = p.rings[t.ring[s]];   /* prefetch of the next visit's record */
             startpix = cur.startpix;
-            bp = beam_pixel_f64(p.hb, p.fc64, c, sd.c2psi, sd.s2psi, ring_phi(p.hs, cur, r, pix),
-                                cur.cdp, cur.sdp, c2, s2);
+            bp = beam_pixel_f64<true>(p.hb, p.fc64, c, sd.c2psi, sd.s2psi,
+                                      ring_phi(p.hs, cur, r, pix), cur.cdp, cur.sdp, c2, s2);
         } else {
             /* the part of the geometry that reads the ring record, then the
              * prefetch of the next visit's record into the same registers */
@@ -287,7 +288,7 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
             bp = beam_pixel_f32_tail(p.fc, sf, g, c2, s2);
         }
         bp = act ? bp : -1;
-        if constexpr (!MAP64) {   /* the double-precision form never defers */
+        {   /* visits whose floors the fast form cannot certify: queued */
             const unsigned long long m = __ballot(bp == kNeedExact);
             if (m) {
                 if (bp == kNeedExact) {
@@ -300,7 +301,7 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
         }
         if (q.live)
             use(q);
-        if constexpr (!MAP64) {
+        {
             /* drain here, where the previous visit is consumed and the next
              * loads are not yet issued: little is live across the exact
              * path's out-of-line call */
@@ -428,8 +429,9 @@ __device__ inline void pair_sample(const PairParams &p, long i, float pa_i, RunT
                                                              use);
         wave_sync();
     }
-    if (!FWD && qn > 0)   /* the forward's mapping never defers */
-        drain_exact<false>(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, use);
+    if (qn > 0)
+        drain_exact<FWD, decltype(use), OP == OP_FWD_V>(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi,
+                                                       use);
     if (FWD) {
         acc_a = wave_sum(acc_a);
         acc_b = wave_sum(acc_b);
@@ -442,8 +444,11 @@ __device__ inline void pair_sample(const PairParams &p, long i, float pa_i, RunT
     }
 }
 
+/* 4 waves per SIMD (128 VGPRs): with the float alpha and its deferral queue
+ * the forward needs about 100 VGPRs; under the 96 of 5 waves per SIMD it
+ * spilled and ran 15% slower at the 5-degree, NSIDE 512 headline. */
 template <int OP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pair(PairParams p)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pair(PairParams p)
 {
     __shared__ RunTable tables[kWavesPerBlock];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
@@ -1206,6 +1211,10 @@ struct AdjLayout {
 constexpr int kAdjWaves = PISCO_ADJ_WAVES;   /* waves per block of the sky adjoint */
 constexpr int kAdjBlock = kAdjWaves * kWaveSize;
 constexpr int kBigI = 0x3fffffff;
+/* A chunk whose window exceeds the LDS capacity is split into at most this
+ * many slabs of consecutive window rings (see k_adjmap_lds). */
+constexpr int kMaxSlabs = 63;
+constexpr int kAdjCtl = 4 + kMaxSlabs + 1;   /* ints of the control block */
 
 /* I,Q,U gradient of one visit, in float: the values are accumulated in float
  * LDS windows, so FP64 products would be rounded away there anyway. */
@@ -1252,11 +1261,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     int *win_lo = (int *)(wtab + kAdjWaves);
     int *win_hi = win_lo + L.rw;
     int *win_base = win_hi + L.rw;          /* rw + 1 */
-    int *ctl = win_base + L.rw + 1;         /* 0 R0, 1 R1, 2 W, 3 fallback */
-    float *acc = (float *)(ctl + 4);
+    int *ctl = win_base + L.rw + 1;         /* 0 R0, 1 R1, 2 W, 3 slabs, 4.. slab starts */
+    int *slab = ctl + 4;                    /* first window ring of each slab, then nring */
+    float *acc = (float *)(ctl + kAdjCtl);
     /* deterministic mode: int64 windows, 8-byte aligned after ctl */
     unsigned long long *acc64 =
-        (unsigned long long *)(((unsigned long)(ctl + 4) + 7ul) & ~7ul);
+        (unsigned long long *)(((unsigned long)(ctl + kAdjCtl) + 7ul) & ~7ul);
     const double fxs = DET ? *L.fx_scale : 0.0;
 
     const int tid = threadIdx.x;
@@ -1310,8 +1320,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             }
         }
         __syncthreads();
-        const int R0 = ctl[0];
-        const int nring = ctl[1] - R0 + 1;
+        const int R0 = __builtin_amdgcn_readfirstlane(ctl[0]);
+        const int nring = __builtin_amdgcn_readfirstlane(ctl[1]) - R0 + 1;
         /* no window ring (a disc of radius rho_w can miss every ring centre
          * while the rho_max discs still have visits) or too many: global
          * atomics for this chunk */
@@ -1394,35 +1404,73 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 }
             }
             __syncthreads();
-            const int W = ctl[2];
-            fallback = W > L.cap;
-            if (!fallback) {
-                for (int k = tid; k < 3 * W; k += kAdjBlock) {
-                    if constexpr (DET)
-                        acc64[k] = 0ull;
-                    else
-                        acc[k] = 0.f;
+            /* slabs: consecutive window rings whose windows add up to at
+             * most cap pixels (one slab, the whole window, when it fits) */
+            if (tid == 0) {
+                const int W = ctl[2];
+                int n = 1;
+                slab[0] = 0;
+                if (W > L.cap) {
+                    int k0 = 0;
+                    n = 0;
+                    while (k0 < nring && n < kMaxSlabs) {
+                        int k1 = k0 + 1;
+                        while (k1 < nring && win_base[k1 + 1] - win_base[k0] <= L.cap)
+                            k1++;
+                        slab[n++] = k0;
+                        k0 = k1;
+                    }
+                    /* beyond kMaxSlabs slabs the last one takes the remaining
+                     * rings: it exceeds cap, and its visits use global atomics */
                 }
+                slab[n] = nring;
+                ctl[3] = n;
             }
             __syncthreads();
         }
+        /* block-uniform slab bounds, read from LDS into scalar registers */
+        const int nslab = fallback ? 1 : __builtin_amdgcn_readfirstlane(ctl[3]);
+        for (int si = 0; si < nslab; si++) {
+        /* this slab: window rings [k0, k1) and their Ws window pixels; the
+         * first slab also walks the disc rings before the window rings and
+         * the last slab those after (their visits take global atomics) */
+        const int k0 = fallback ? 0 : __builtin_amdgcn_readfirstlane(slab[si]);
+        const int k1 = fallback ? 0 : __builtin_amdgcn_readfirstlane(slab[si + 1]);
+        const int base0 = fallback ? 0 : __builtin_amdgcn_readfirstlane(win_base[k0]);
+        const int Ws = fallback ? 0 : __builtin_amdgcn_readfirstlane(win_base[k1]) - base0;
+        const bool sfall = fallback || Ws > L.cap;
+        const int rlo = (fallback || si == 0) ? -kBigI : R0 + k0;
+        const int rhi = (fallback || si == nslab - 1) ? kBigI : R0 + k1 - 1;
+        auto clamp = [&](Disc &d) {
+            d.r_first = d.r_first > rlo ? d.r_first : rlo;
+            d.r_last = d.r_last < rhi ? d.r_last : rhi;
+        };
+        if (!sfall) {
+            for (int k = tid; k < 3 * Ws; k += kAdjBlock) {
+                if constexpr (DET)
+                    acc64[k] = 0ull;
+                else
+                    acc[k] = 0.f;
+            }
+        }
+        __syncthreads();
 
         /* pass 2: visits; adds I,Q,U of one visit to the LDS window (or, when no
          * window holds the pixel, to the global FP64 gradient) */
-        const int W = fallback ? 0 : ctl[2];
+        const int W = sfall ? 0 : Ws;
         auto scatter = [&](int pix, int r, int sp, float vI, float vQ, float vU) {
             if (vI == 0.f && vQ == 0.f && vU == 0.f)
                 return;   /* adds nothing */
             const int k = r - R0;
             int slot = -1;
-            if (!fallback && (unsigned)k < (unsigned)nring) {
+            if (!sfall && k >= k0 && k < k1) {
                 int nr = 4 * (r < p.hs.nside ? r
                               : (r > 3 * p.hs.nside ? 4 * p.hs.nside - r : p.hs.nside));
                 int off = pix - sp - win_lo[k];
                 if (off < 0)
                     off += nr;
                 if (off < win_hi[k])   /* inside the ring's window */
-                    slot = win_base[k] + off;
+                    slot = win_base[k] - base0 + off;
             }
             if constexpr (DET) {
                 const unsigned long long fI = (unsigned long long)(long long)rint((double)vI * fxs);
@@ -1468,6 +1516,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 Disc d;
                 disc_setup(p.hs, kHalfPi - c.dec, c.ra, L.pairs[d0].rho_max,
                            L.pairs[d0].cos_rmax, d);
+                clamp(d);
                 auto consume = [&](const BatchPend<BG> &q) {
                     float vI = 0.f, vQ = 0.f, vU = 0.f;
 #pragma unroll
@@ -1504,7 +1553,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 Centre c;
                 double c2psi, s2psi;
                 Disc d;
-                const int nrings = sub_setup(p, i, valid, c, c2psi, s2psi, d);
+                int nrings = sub_setup(p, i, valid, c, c2psi, s2psi, d);
+                clamp(d);
+                nrings = nrings > 0 && d.r_last >= d.r_first ? d.r_last - d.r_first + 1 : 0;
                 const float ga = (valid && p.g_a) ? p.g_a[i] : 0.f;
                 const float gb = (valid && p.g_b) ? p.g_b[i] : 0.f;
                 auto use = [&](const Pending &q) {
@@ -1543,6 +1594,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             SampleD sd;
             Disc d;
             sample_setup(pp, i, pa_i, c, sf, sd, d);
+            clamp(d);
             const float ga = unif(pp.g_a ? pp.g_a[i] : 0.f);
             const float gb = unif(pp.g_b ? pp.g_b[i] : 0.f);
             auto use = [&](const Pending &q) {
@@ -1562,8 +1614,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
         }
         __syncthreads();
 
-        /* flush the window */
-        if (!fallback) {
+        /* flush the slab's window */
+        if (!sfall) {
             for (int slot = tid; slot < W; slot += kAdjBlock) {
                 float vI = 0.f, vQ = 0.f, vU = 0.f;
                 unsigned long long fI = 0, fQ = 0, fU = 0;
@@ -1580,10 +1632,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                     if (vI == 0.f && vQ == 0.f && vU == 0.f)
                         continue;
                 }
-                int lo = 0, hi = nring;   /* win_base[lo] <= slot < win_base[hi] */
+                int lo = k0, hi = k1;   /* win_base[lo] <= base0 + slot < win_base[hi] */
                 while (hi - lo > 1) {
                     int mid = (lo + hi) >> 1;
-                    if (win_base[mid] <= slot)
+                    if (win_base[mid] - base0 <= slot)
                         lo = mid;
                     else
                         hi = mid;
@@ -1591,7 +1643,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
                 int sp, nr;
                 bool shf;
                 ring_info(p.hs, R0 + lo, sp, nr, shf);
-                int ip = win_lo[lo] + (slot - win_base[lo]);
+                int ip = win_lo[lo] + (slot - (win_base[lo] - base0));
                 if (ip >= nr)
                     ip -= nr;
                 int pix = sp + ip;
@@ -1613,6 +1665,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
             }
         }
         __syncthreads();
+        }   /* slabs */
     }
 }
 
@@ -2158,12 +2211,16 @@ __global__ void k_disc(Hpx h, long n, const float *ra, const float *dec, double 
 /* Sky pixel -> (beam pixel, chi) for pixel lists laid out as offsets/pixels
  * (the quantity kernel 1 of the reference computes, cudaConv2.cu:59-128).
  * mode 0: the exact FP64 mapping, chi[k] = chi.
- * mode 1: what the convolution kernels use: the float form, and the exact
+ * mode 1: what the adjoint kernels use: the float form, and the exact
  *         mapping where the float form cannot certify the pixel; chi[k]
  *         receives float cos(2chi), sin(2chi) packed as two floats.
  * mode 2: the float form alone; bpix[k] = kNeedExact (-2) where it declines.
- * mode 3: the certified double-precision form of the forward kernel (output
- *         as mode 1). */
+ * mode 3: the certified double-precision form of the small-disc kernels and
+ *         of the deferred-visit drain (output as mode 1).
+ * mode 4: what the one-wave-per-sample forward uses: the double-precision
+ *         form with the float alpha, and mode 3 where it declines.
+ * mode 5: the double-precision form with the float alpha alone; bpix[k] =
+ *         kNeedExact where it declines. */
 __global__ void k_match(Hpx hs, const RingRec *rings, Hpx hb, FastConsts fc, FastConsts64 fc64,
                         int mode, long n,
                         const float *ra, const float *dec, const float *pa,
@@ -2183,9 +2240,17 @@ __global__ void k_match(Hpx hs, const RingRec *rings, Hpx hb, FastConsts fc, Fas
         if (mode > 0) {
             float c2 = 0.f, s2 = 0.f;
             double ra_pix = ring_phi(hs, rr, r, pix);
-            int bp = (mode == 3)
-                         ? beam_pixel_f64(hb, fc64, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2)
-                         : beam_pixel_f32(fc, c, sf, ra_pix, rr.cdp, rr.sdp, c2, s2);
+            int bp;
+            if (mode >= 4) {
+                bp = beam_pixel_f64<true>(hb, fc64, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2,
+                                          s2);
+                if (bp == kNeedExact && mode == 4)
+                    bp = beam_pixel_f64(hb, fc64, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
+            } else if (mode == 3) {
+                bp = beam_pixel_f64(hb, fc64, c, c2psi, s2psi, ra_pix, rr.cdp, rr.sdp, c2, s2);
+            } else {
+                bp = beam_pixel_f32(fc, c, sf, ra_pix, rr.cdp, rr.sdp, c2, s2);
+            }
             if (bp == kNeedExact && mode == 1) {
                 ExactVisit e =
                     beam_visit_exact(hb.nside, c.ra, c.psi, c.cdc, c.sdc, ra_pix, rr.cdp, rr.sdp);
@@ -2337,7 +2402,7 @@ size_t adjmap_lds_bytes(int cap, int rw, int group, bool det)
     const size_t wt = group == 2 ? sizeof(BatchTable<2>)
                       : group == 4 ? sizeof(BatchTable<4>)
                                    : sizeof(RunTable);
-    return wt * kAdjWaves + sizeof(int) * (3 * (size_t)rw + 1 + 4) +
+    return wt * kAdjWaves + sizeof(int) * (3 * (size_t)rw + 1 + kAdjCtl) +
            (det ? 8 + 8 * 3 * (size_t)cap : sizeof(float) * 3 * (size_t)cap);
 }
 

@@ -139,7 +139,9 @@ def test_match_fast_path_bit_exact(gpu, oracle, nside_sky, nside_beam, cap_deg):
     single-precision error bound (9.2e-7 measured in float32 emulation;
     tolerance 2e-6).  The single-precision form alone declines (defers to
     the exact path) on a small fraction of visits, and every pixel it does
-    certify is the oracle's."""
+    certify is the oracle's.  The same for the forward's double-precision
+    form with a float atan2 for alpha (floors certified against its 1.5e-6
+    rad bound, declined visits mapped by the FP64-alpha form)."""
     rng = np.random.default_rng(17)
     n = 600 if nside_sky <= 512 else 3000
     ra = rng.uniform(0, 2 * math.pi, n).astype(np.float32)
@@ -160,6 +162,17 @@ def test_match_fast_path_bit_exact(gpu, oracle, nside_sky, nside_beam, cap_deg):
     assert np.array_equal(b_h, b_3)
     assert np.max(np.abs(c3 - c2_ref)) <= 2.5e-7
     assert np.max(np.abs(s3 - s2_ref)) <= 2.5e-7
+    # the one-wave-per-sample forward's form: float alpha, mode 3 where it
+    # declines; and that form alone, which must decline rarely
+    b_4, c4, s4 = gpu.match_fast(nside_sky, nside_beam, ra, dec, pa, off, pix, mode=4)
+    assert np.array_equal(b_h, b_4)
+    assert np.max(np.abs(c4 - c2_ref)) <= 2.5e-7
+    assert np.max(np.abs(s4 - s2_ref)) <= 2.5e-7
+    b_5, _, _ = gpu.match_fast(nside_sky, nside_beam, ra, dec, pa, off, pix, mode=5)
+    cert5 = b_5 != -2
+    assert np.array_equal(b_5[cert5], b_h[cert5])
+    print("float-alpha form declined %.2e" % (1.0 - cert5.mean()))
+    assert 1.0 - cert5.mean() < 0.02
     b_1, _, _ = gpu.match_fast(nside_sky, nside_beam, ra, dec, pa, off, pix, mode=2)
     cert = b_1 != -2
     assert np.array_equal(b_1[cert], b_h[cert])
