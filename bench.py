@@ -24,8 +24,9 @@ the dominant kernel / its mean duration (HIP events on the launch stream).
                   [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker] [--c5]
                   [--beam-adjoint-samples M] [--calib-copy-mib M]
 
-The beam adjoint (TOD -> beam maps) is timed on a prefix of the headline scan
-(default 2.5e5 samples), with B_adjb = sum_i |D_i| * 60 + N * 20.
+The beam adjoint (TOD -> beam maps) is timed over the headline scan (or the
+prefix --beam-adjoint-samples), with B_adjb = sum_i |D_i| * 60 + N * 20; the
+annulus-band form it replaced is timed beside it on the first 2.5e5 samples.
 
 Also reported: the plain drop-in forward of the NSIDE 256 / 1.5 deg workload
 (device-resident and through the host API including PCIe copies), the
@@ -79,8 +80,9 @@ def parse():
                    help="skip the batched 64-detector workload")
     p.add_argument("--c5", action="store_true",
                    help="also time the large configuration (NSIDE 2048, 256 detectors)")
-    p.add_argument("--beam-adjoint-samples", type=int, default=250_000,
-                   help="prefix of the headline scan on which the beam adjoint is timed (0: skip)")
+    p.add_argument("--beam-adjoint-samples", type=int, default=-1,
+                   help="prefix of the headline scan on which the beam adjoint is timed "
+                        "(-1: the whole scan; 0: skip)")
     p.add_argument("--no-mapmaker", action="store_true",
                    help="skip the binned map-maker workload")
     p.add_argument("--calib-copy-mib", type=int, default=0,
@@ -199,28 +201,43 @@ def main():
     check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 0))
 
     # the beam adjoint (TOD -> beam maps) on a prefix, outside the timed steps
-    nbp = min(n, args.beam_adjoint_samples)
+    nbp = n if args.beam_adjoint_samples < 0 else min(n, args.beam_adjoint_samples)
     beam_adj = None
     if nbp > 0:
         gbeam8 = torch.empty(npb * 8, dtype=torch.float32, device=dev)
 
-        def adjb():
+        def adjb(m):
             check(lib.pisco_dev_adjoint_beam(ctx, stream, nside_sky, It.data_ptr(), Qt.data_ptr(),
-                                             Ut.data_ptr(), nside_beam, npb, nbp, rat.data_ptr(),
+                                             Ut.data_ptr(), nside_beam, npb, m, rat.data_ptr(),
                                              dect.data_ptr(), pat.data_ptr(), gat.data_ptr(),
                                              gbt.data_ptr(), gbeam8.data_ptr()))
-        adjb()
-        d0.record()
-        adjb()
-        d1.record()
-        torch.cuda.synchronize()
-        t_adjb = d0.elapsed_time(d1) / 1e3
-        v_pre = int(counts[:nbp].to(torch.int64).sum().item())
-        b_adjb = v_pre * (12 + 24 * k) + nbp * (12 + 4 * k)
+
+        def time_adjb(m):
+            adjb(m)
+            d0.record()
+            adjb(m)
+            d1.record()
+            torch.cuda.synchronize()
+            return d0.elapsed_time(d1) / 1e3
+
+        def model(m):
+            v_pre = int(counts[:m].to(torch.int64).sum().item())
+            return v_pre * (12 + 24 * k) + m * (12 + 4 * k)
+        t_adjb = time_adjb(nbp)
+        b_adjb = model(nbp)
         beam_adj = {"samples": nbp, "s": t_adjb, "tod_samples_per_s": k * nbp / t_adjb,
                     "hbm_frac": b_adjb / t_adjb / HBM_PEAK,
-                    "note": "LDS bands of beam rows, 8 samples per wave; the visit walks "
-                            "of the band annuli dominate"}
+                    "form": "visit records in band order (two mapping passes), band sums in "
+                            "LDS as 64-bit fixed point"}
+        # the annulus-band form, on a prefix
+        nba = min(nbp, 250_000)
+        check(lib.pisco_set_tuning(ctx, b"adjbeam_records", 0))
+        t_band = time_adjb(nba)
+        check(lib.pisco_set_tuning(ctx, b"adjbeam_records", 1))
+        t_rec = time_adjb(nba)
+        beam_adj["annulus_bands"] = {"samples": nba, "s": t_band,
+                                     "hbm_frac": model(nba) / t_band / HBM_PEAK,
+                                     "records_s_same_prefix": t_rec}
         del gbeam8
 
     dominant = "adjoint_map" if t_adj >= t_fwd else "forward"

@@ -146,10 +146,17 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * table, sorted by disc radius: k > 0 (a multiple of 4) pairs per range, -1
  * one range of all pairs; 0, the default, ranges whose packed beams add up
  * to at most 4 MiB), "adjbeam_bands" (beam adjoint: 1, the default, sums in
- * LDS bands of beam rows over annuli of the discs; 0 global FP64 atomics),
- * "adjbeam_lanes" (banded beam adjoint: 8, the default, lanes per sample
+ * LDS bands of beam rows; 0 global FP64 atomics), "adjbeam_records" (with
+ * adjbeam_bands 1: 1, the default, maps each visit twice -- counting, then
+ * writing a 16-byte record per visit in the order of its band of beam rows
+ * -- and sums the records band by band in LDS as 64-bit fixed-point
+ * integers, which makes the result independent of the order of the
+ * additions (beams of up to 512 bands); 0 walks each sample's annulus
+ * once per band), "adjbeam_pool_mb" (memory of the records, which sets the
+ * samples per chunk, MiB; default 8192),
+ * "adjbeam_lanes" (annulus walks: 8, the default, lanes per sample
  * with 8 samples per wave, or 64),
- * "adjbeam_chunk" (samples per banded work item, default 1024),
+ * "adjbeam_chunk" (samples per annulus work item, default 1024),
  * "adjbeam_rows" (rows per band; 0, the default, as many as fit the LDS),
  * "v_term" (host-buffer calls: 1 adds the reference GPU kernel's V A3 and
  * V B3 (cudaConv2.cu:192,204,214,217-218) to pisco_exec_convolution's TOD,

@@ -183,17 +183,24 @@ struct pisco_ctx {
     int adjbeam_chunk = 1024;     /* samples per work item of the banded beam adjoint */
     int adjbeam_rows = 0;         /* beam rows per band; 0 = as many as fit the LDS */
     int adjbeam_lanes = 8;        /* lanes per sample of the banded beam adjoint: 8 or 64 */
+    int adjbeam_records = 1;      /* beam adjoint through visit records (1) or the bands (0) */
+    int adjbeam_pool_mb = 8192;   /* memory of the records per chunk of samples, MiB */
     int v_term = 0;               /* 1: host-API TOD adds V A3, V B3 as the reference's GPU kernel */
     DevBuf<unsigned> fxwork;      /* deterministic mode: max row sum, max |g| */
     DevBuf<double> fxscale;       /* deterministic mode: the scale (device) */
     DevBuf<int> support;          /* last non-zero beam row (device) */
     DevBuf<float> sky4;           /* the forward's packed sky: I, Q, U, 0 per pixel */
+    DevBuf<float4> recs;          /* beam adjoint: visit records (16 bytes each) */
+    DevBuf<long> recidx;          /* beam adjoint: record offsets per sample and band */
+    DevBuf<float> recvmax;        /* beam adjoint: bounds of the record values */
+    DevBuf<double> recpart;       /* beam adjoint: partial band sums */
     bool frozen = false;          /* pisco_freeze_buffers */
 
     template <typename F> void each_buf(F f)
     {
         f(I); f(Q); f(U); f(V); f(beam); f(ra); f(dec); f(pa); f(tod); f(cot); f(acc); f(fout);
-        f(fxwork); f(fxscale); f(support); f(sky4);
+        f(fxwork); f(fxscale); f(support); f(sky4); f(recs); f(recidx);
+        f(recvmax); f(recpart);
     }
 };
 
@@ -515,6 +522,14 @@ int pisco_set_tuning(pisco_ctx *c, const char *key, long value)
         if (value < 0 || value > (1L << 30))
             return fail(PISCO_ERR_INVALID, "adjbeam_rows out of range");
         c->adjbeam_rows = (int)value;
+    } else if (k == "adjbeam_records") {
+        if (value != 0 && value != 1)
+            return fail(PISCO_ERR_INVALID, "adjbeam_records must be 0 or 1");
+        c->adjbeam_records = (int)value;
+    } else if (k == "adjbeam_pool_mb") {
+        if (value < 1 || value > (1L << 20))
+            return fail(PISCO_ERR_INVALID, "adjbeam_pool_mb must be in [1, 2^20]");
+        c->adjbeam_pool_mb = (int)value;
     } else if (k == "adjbeam_chunk") {
         if (value < 1 || value > (1L << 24))
             return fail(PISCO_ERR_INVALID, "adjbeam_chunk must be in [1, 2^24]");
@@ -564,6 +579,11 @@ int pisco_get_tuning(pisco_ctx *c, const char *key, long *value)
         *value = c->adjbeam_rows;
     else if (k == "adjbeam_lanes")
         *value = c->adjbeam_lanes;
+    else if (k == "adjbeam_records")
+        *value = c->adjbeam_records;
+    else if (k == "adjbeam_pool_mb")
+        *value = c->adjbeam_pool_mb;
+
     else if (k == "adjmap_cap")
         *value = c->adjmap_cap;
     else if (k == "adjmap_rw")
@@ -1134,8 +1154,20 @@ int pisco_dev_adjoint_beam(pisco_ctx *c, void *stream, int nside_sky, const floa
         CHK(c->sky4.ensure(4 * (size_t)npix));
         HIPCHK(launch_pack_sky(I, Q, U, npix, (float4 *)c->sky4.p, st));
         p.sky4 = (const float4 *)c->sky4.p;
-        hipError_t e = launch_adjbeam_bands(p, c->adjbeam_chunk, c->adjbeam_rows,
-                                            c->adjmap_blocks, c->ncu, c->adjbeam_lanes, st);
+        RecPlan pl;
+        hipError_t e;
+        if (c->adjbeam_records &&
+            adjbeam_rec_plan(p, c->adjbeam_rows, (size_t)c->adjbeam_pool_mb << 20, c->ncu, pl)) {
+            CHK(c->recs.ensure(pl.nrec));
+            CHK(c->recidx.ensure(pl.nidx));
+            CHK(c->recvmax.ensure(2));
+            CHK(c->recpart.ensure(pl.npart));
+            e = launch_adjbeam_records(p, pl, c->recs.p, c->recidx.p, c->recvmax.p,
+                                       c->recpart.p, st);
+        } else {
+            e = launch_adjbeam_bands(p, c->adjbeam_chunk, c->adjbeam_rows, c->adjmap_blocks,
+                                     c->ncu, c->adjbeam_lanes, st);
+        }
         if (e != hipSuccess)
             return fail(PISCO_ERR_DEVICE, "beam adjoint launch failed: %s",
                         hipGetErrorString(e));

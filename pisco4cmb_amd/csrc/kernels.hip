@@ -2023,6 +2023,389 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band(PairParams p, BeamBa
     }
 }
 
+/* ---- beam adjoint through visit records -----------------------------------
+ *
+ * The banded kernels above map each sample's annulus once per band of beam
+ * rows and set up every (band, sample) pair.  Here each visit is mapped
+ * twice, as in the forward, and the gradient is formed from 16-byte visit
+ * records, per chunk of samples:
+ *  1. k_beamrec<false> (one wave per sample, the forward's walk and mapping)
+ *     counts the sample's visits inside the beam per band of `rows`
+ *     consecutive beam rows;
+ *  2. k_band_scan + k_scan_excl: per band, the offsets of the samples' records
+ *     within the band; the bands' starts;
+ *  3. k_beamrec<true> maps the visits again and writes each one's record
+ *     (row within the band | sample << kRecRowBits, I, X = Q c2 + U s2,
+ *     Y = -Q s2 + U c2) at its place in band order;
+ *  4. k_beamrec_sum: a block per (band, part of the band's records) sums its
+ *     records, times the cotangents of each record's sample, into the band's
+ *     rows in LDS as 64-bit fixed-point integers, then adds them to its own
+ *     FP64 partial rows (a (band, part) slice belongs to one block in every
+ *     chunk: no global atomics).
+ * k_beamrec_reduce adds the parts into the FP64 gradient at the end.
+ * Integers because float atomic adds to random LDS addresses ran about ten
+ * times slower than 64-bit integer ones here (6.8 against 1.0 ms per chunk
+ * of 41,000 samples at the 5-degree, beam NSIDE 4096 shape); integer sums
+ * also make the result independent of the order of the additions. */
+struct BeamRec {
+    unsigned key;   /* row within the band | sample within the chunk << kRecRowBits */
+    float sI, X, Y;
+};
+constexpr int kRecRowBits = 13;
+constexpr int kRecUnroll = 4;
+constexpr int kScanBlock = 1024;
+
+/* In-place exclusive scan of x[0 .. n) (one block); x[n] receives the total. */
+__global__ __launch_bounds__(kScanBlock) void k_scan_excl(long *x, long n)
+{
+    __shared__ long part[kScanBlock];
+    const int t = threadIdx.x;
+    const long seg = (n + kScanBlock - 1) / kScanBlock;
+    const long a = std::min<long>(n, t * seg), b = std::min<long>(n, a + seg);
+    long s = 0;
+    for (long k = a; k < b; k++)
+        s += x[k];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < kScanBlock; o <<= 1) {
+        const long v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long run = part[t] - s;
+    for (long k = a; k < b; k++) {
+        const long v = x[k];
+        x[k] = run;
+        run += v;
+    }
+    if (t == kScanBlock - 1)
+        x[n] = part[t];
+}
+
+__device__ inline int rec_band(const RecParams &R, int bp)
+{
+    int b = (int)((float)bp * R.inv_rows);
+    if (b * R.rows > bp)
+        b--;
+    else if ((b + 1) * R.rows <= bp)
+        b++;
+    return b;
+}
+
+/* Counts (WRITE false) or writes (WRITE true) the live lanes' records;
+ * called by the whole wave.  cur: per band, the count, or the next position
+ * in band order. */
+template <bool WRITE>
+struct RecSink {
+    const RecParams &R;
+    unsigned *cur;
+    unsigned smp_key;
+    __device__ inline void operator()(bool live, int bp, float c2, float s2, float4 sv)
+    {
+        if (!live)
+            return;
+        const int b = rec_band(R, bp);
+        if constexpr (WRITE) {
+            const unsigned slot = atomicAdd(&cur[b], 1u);
+            if (slot >= (unsigned)R.nrec_cap)
+                return;   /* never: both passes count the same visits */
+            BeamRec r;
+            r.key = (unsigned)(bp - b * R.rows) | smp_key;
+            r.sI = sv.x;
+            r.X = sv.y * c2 + sv.z * s2;
+            r.Y = -sv.y * s2 + sv.z * c2;
+            R.brec[slot] = r;
+        } else {
+            atomicAdd(&cur[b], 1u);
+        }
+    }
+};
+
+/* Maps the deferred visits base .. base+cnt-1 (cnt <= 64, one per lane) with
+ * the FP64-alpha form and hands them to the sink (whole wave). */
+template <typename Sink>
+__device__ inline void drain_records(const PairParams &p, RunTable &t, int base, int cnt, int lane,
+                                     const Centre &c, double c2psi, double s2psi, Sink &sink)
+{
+    wave_sync();
+    bool live = false;
+    int bp = -1;
+    float c2 = 0.f, s2 = 0.f;
+    float4 sv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < cnt) {
+        const int pix = t.qpix[base + lane], r = t.qring[base + lane];
+        const RingRec rr = p.rings[r];
+        bp = beam_pixel_f64(p.hb, p.fc64, c, c2psi, s2psi, ring_phi(p.hs, rr, r, pix), rr.cdp,
+                            rr.sdp, c2, s2);
+        live = (unsigned)bp < (unsigned)p.npix_beam;
+        if (live && Sink::kSky)
+            sv = p.sky4[pix];
+    }
+    sink(live, bp, c2, s2, sv);
+    __builtin_amdgcn_s_waitcnt(0);   /* as in drain_exact */
+    wave_sync();
+}
+
+/* The forward's walk (walk_visits with the FP64 form and its float alpha)
+ * for records: no beam rows are read; with SKY, each visit's sky values are
+ * loaded at the end of its iteration and used in the next one. */
+template <bool SKY, typename Sink>
+__device__ inline void walk_records(const PairParams &p, RunTable &t, int total, int lane,
+                                    const Centre &c, const SampleD &sd, int &qn, Sink &sink)
+{
+    if (total <= 0)
+        return;
+    int s = 0;
+    int v = lane;
+    if (v < total)
+        while (v >= t.pref[s + 1])
+            s++;
+    RingRec rr = p.rings[t.ring[s]];
+    __builtin_amdgcn_s_waitcnt(0);   /* as in walk_visits */
+    bool qlive = false;
+    int qbp = -1;
+    float qc2 = 0.f, qs2 = 0.f;
+    float4 qsv = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
+        const bool act = v < total;
+        const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
+        const int r = t.ring[s];
+        const RingRec cur = rr;
+        const int vn = v + kWaveSize;
+        if (vn < total)
+            while (vn >= t.pref[s + 1])
+                s++;
+        rr = p.rings[t.ring[s]];
+        float c2, s2;
+        int bp = beam_pixel_f64<true>(p.hb, p.fc64, c, sd.c2psi, sd.s2psi,
+                                      ring_phi(p.hs, cur, r, pix), cur.cdp, cur.sdp, c2, s2);
+        bp = act ? bp : -1;
+        {
+            const unsigned long long m = __ballot(bp == kNeedExact);
+            if (m) {
+                if (bp == kNeedExact) {
+                    const int pos = qn + lanes_below(m);
+                    t.qpix[pos] = pix;
+                    t.qring[pos] = r;
+                }
+                qn += __popcll(m);
+            }
+        }
+        sink(qlive, qbp, qc2, qs2, qsv);
+        if (qn >= kWaveSize) {
+            drain_records(p, t, qn - kWaveSize, kWaveSize, lane, c, sd.c2psi, sd.s2psi, sink);
+            qn -= kWaveSize;
+        }
+        qlive = (unsigned)bp < (unsigned)p.npix_beam;
+        qbp = bp;
+        qc2 = c2;
+        qs2 = s2;
+        if constexpr (SKY)
+            qsv = p.sky4[qlive ? pix : 0];
+    }
+    sink(qlive, qbp, qc2, qs2, qsv);
+}
+
+template <bool WRITE>
+struct RecSinkT : RecSink<WRITE> {
+    static constexpr bool kSky = WRITE;
+};
+
+/* Steps 1 and 3: one wave per sample of the chunk [p.s_begin, p.s_end). */
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_beamrec(PairParams p, RecParams R)
+{
+    __shared__ RunTable tables[kWavesPerBlock];
+    __shared__ unsigned curs[kWavesPerBlock][kRecMaxBands];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
+    const int lane = threadIdx.x % kWaveSize;
+    RunTable &t = tables[wave];
+    unsigned *cur = curs[wave];
+    const long j = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    const long i = p.s_begin + j;
+    if (i >= p.s_end)
+        return;
+    for (int b = lane; b < R.nband; b += kWaveSize)
+        cur[b] = WRITE ? (unsigned)(R.bbase[b] + R.bcnt[b * R.chunk + j]) : 0u;
+    wave_sync();
+    RecSinkT<WRITE> sink{{R, cur, (unsigned)j << kRecRowBits}};
+    const float ga = unif(p.g_a ? p.g_a[i] : 0.f);
+    const float gb = unif(p.g_b ? p.g_b[i] : 0.f);
+    if (ga != 0.f || gb != 0.f) {   /* else it adds nothing (wave-uniform) */
+        Centre c;
+        SampleF sf;
+        SampleD sd;
+        Disc d;
+        sample_setup(p, i, p.pa[i], c, sf, sd, d);
+        int qn = 0;
+        for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+            const int total = build_runs(p.hs, d, r0, lane, t);
+            walk_records<WRITE>(p, t, total, lane, c, sd, qn, sink);
+            wave_sync();
+        }
+        if (qn > 0)
+            drain_records(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, sink);
+    }
+    if constexpr (!WRITE) {
+        wave_sync();
+        for (int b = lane; b < R.nband; b += kWaveSize)
+            R.bcnt[b * R.chunk + j] = cur[b];
+    }
+}
+
+/* Step 2: per band (one block each), the exclusive offsets of the samples'
+ * records within the band (in place) and the band's total in bbase[b].
+ * Each thread scans kScanItems consecutive samples per round. */
+constexpr int kScanItems = 16;
+__global__ __launch_bounds__(kScanBlock) void k_band_scan(RecParams R, long nsamp)
+{
+    __shared__ long part[kScanBlock];
+    const int b = blockIdx.x, t = threadIdx.x;
+    long *x = R.bcnt + (long)b * R.chunk;
+    long carry = 0;
+    for (long r0 = 0; r0 < nsamp; r0 += (long)kScanBlock * kScanItems) {
+        long v[kScanItems];
+        long s = 0;
+#pragma unroll
+        for (int q = 0; q < kScanItems; q++) {
+            const long k = r0 + (long)t * kScanItems + q;
+            v[q] = k < nsamp ? x[k] : 0;
+            s += v[q];
+        }
+        part[t] = s;
+        __syncthreads();
+        for (int o = 1; o < kScanBlock; o <<= 1) {
+            const long w = t >= o ? part[t - o] : 0;
+            __syncthreads();
+            part[t] += w;
+            __syncthreads();
+        }
+        long run = carry + part[t] - s;
+#pragma unroll
+        for (int q = 0; q < kScanItems; q++) {
+            const long k = r0 + (long)t * kScanItems + q;
+            if (k < nsamp)
+                x[k] = run;
+            run += v[q];
+        }
+        carry += part[kScanBlock - 1];
+        __syncthreads();
+    }
+    if (t == 0)
+        R.bbase[b] = carry;
+}
+
+/* Step 4: block (band, part) of nsplit.  Each value v = g * (I, X or Y) is
+ * rounded once to a multiple of 2^-k and summed as a 64-bit integer, with k
+ * the largest integer such that the block's records, each |v| <= gmax *
+ * smax * (1 + 1e-3), cannot sum past 2^62 in magnitude. */
+__global__ __launch_bounds__(kAdjBlock) void k_beamrec_sum(PairParams p, RecParams R)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long *acc = (unsigned long long *)smem;
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x / R.nsplit, part = blockIdx.x % R.nsplit;
+    const int lo = b * R.rows;
+    const int nrow = min(p.npix_beam - lo, R.rows);
+    const long r0 = R.bbase[b], cnt = R.bbase[b + 1] - r0;
+    const long per = (cnt + R.nsplit - 1) / R.nsplit;
+    const long k0 = std::min(cnt, part * per), j1 = std::min(cnt, (part + 1) * per);
+    if (k0 >= j1)
+        return;   /* no records (uniform per block) */
+    const double vmax = (double)R.vmax[0] * (double)R.vmax[1] * (1.0 + 1e-3);
+    if (!(vmax > 0.0))
+        return;   /* every value is zero */
+    const int e = (int)std::ceil(std::log2((double)(j1 - k0) * vmax));
+    const double scale = std::ldexp(1.0, 62 - e);
+    for (int k = tid; k < 6 * nrow; k += kAdjBlock)
+        acc[k] = 0ull;
+    __syncthreads();
+    for (long kk = k0; kk < j1; kk += (long)kRecUnroll * kAdjBlock) {
+        BeamRec r[kRecUnroll];
+        float ga[kRecUnroll], gb[kRecUnroll];
+#pragma unroll
+        for (int u = 0; u < kRecUnroll; u++) {
+            const long k = kk + u * kAdjBlock + tid;
+            r[u] = R.brec[r0 + (k < j1 ? k : k0)];
+            if (k >= j1)
+                r[u].sI = r[u].X = r[u].Y = 0.f;   /* adds zeros */
+        }
+#pragma unroll
+        for (int u = 0; u < kRecUnroll; u++) {
+            const long i = p.s_begin + (long)(r[u].key >> kRecRowBits);
+            ga[u] = p.g_a ? p.g_a[i] : 0.f;
+            gb[u] = p.g_b ? p.g_b[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kRecUnroll; u++) {
+            unsigned long long *a = acc + 6 * (int)(r[u].key & ((1u << kRecRowBits) - 1));
+            const float v[6] = {ga[u] * r[u].sI, ga[u] * r[u].X,    ga[u] * r[u].Y,
+                                gb[u] * r[u].sI, -(gb[u] * r[u].X), -(gb[u] * r[u].Y)};
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+                if (v[m] != 0.f)
+                    atomicAdd(&a[m], (unsigned long long)(long long)rint((double)v[m] * scale));
+        }
+    }
+    __syncthreads();
+    const double inv = 1.0 / scale;
+    double *dst = R.part + ((long)part * p.npix_beam + lo) * 6;
+    for (int k = tid; k < 6 * nrow; k += kAdjBlock)
+        dst[k] += (double)(long long)acc[k] * inv;
+}
+
+/* The parts' sums into the FP64 gradient (rows of eight, columns 0..5), in
+ * part order. */
+__global__ void k_beamrec_reduce(const double *part, int nsplit, long npix, double *gbeam)
+{
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < 6 * npix; k += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < nsplit; q++)
+            s += part[(long)q * 6 * npix + k];
+        const long row = k / 6;
+        gbeam[8 * row + (k - 6 * row)] += s;
+    }
+}
+
+/* vmax[0] = max |g_a|, |g_b| over the samples; vmax[1] = max over sky
+ * pixels of max(|I|, |Q| + |U|) (bounds |I|, |X| and |Y| of every record).
+ * vmax must hold zeros on entry; non-negative floats order as their bits. */
+__global__ void k_rec_vmax(PairParams p, float *vmax)
+{
+    float g = 0.f, sk = 0.f;
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = p.s_end - p.s_begin, stride = (long)gridDim.x * blockDim.x;
+    for (long j = k; j < n; j += stride) {
+        if (p.g_a)
+            g = fmaxf(g, fabsf(p.g_a[p.s_begin + j]));
+        if (p.g_b)
+            g = fmaxf(g, fabsf(p.g_b[p.s_begin + j]));
+    }
+    for (long j = k; j < p.hs.npix; j += stride) {
+        const float4 v = p.sky4[j];
+        sk = fmaxf(sk, fmaxf(fabsf(v.x), fabsf(v.y) + fabsf(v.z)));
+    }
+#pragma unroll
+    for (int m = kWaveSize / 2; m > 0; m >>= 1) {
+        g = fmaxf(g, __shfl_xor(g, m, kWaveSize));
+        sk = fmaxf(sk, __shfl_xor(sk, m, kWaveSize));
+    }
+    if (threadIdx.x % kWaveSize == 0) {
+        atomicMax((unsigned *)&vmax[0], __float_as_uint(g));
+        atomicMax((unsigned *)&vmax[1], __float_as_uint(sk));
+    }
+}
+
+__global__ void k_fill_f64_n(double *x, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
 __global__ void k_fill_i32(int *x, int v) { *x = v; }
 
 /* Last row with a non-zero value in columns 0..5 (wave max, one atomic per
@@ -2535,6 +2918,86 @@ hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nb
                            B);
     else
         hipLaunchKernelGGL(k_adjbeam_band, dim3((unsigned)g), dim3(kAdjBlock), bytes, st, p, B);
+    return hipGetLastError();
+}
+
+bool adjbeam_rec_plan(const PairParams &p, int rows, size_t pool_bytes, int ncu, RecPlan &pl)
+{
+    const long n = p.s_end - p.s_begin;
+    /* six 64-bit sums per row in the band sums' LDS */
+    const int max_rows = std::min<int>(1 << kRecRowBits, 163840 / (6 * 8));
+    pl.rows = rows > 0 && rows < max_rows ? rows : max_rows;
+    pl.rows = (int)std::max<long>(1, std::min<long>(pl.rows, p.npix_beam));
+    pl.nband = (int)((p.npix_beam + pl.rows - 1) / pl.rows);
+    if (pl.nband > kRecMaxBands || n <= 0)
+        return false;
+    /* Visits per sample, at most: the sky pixels whose centres lie within
+     * rho_max are disjoint, each of area 4 pi / npix, and lie inside the cap
+     * of radius rho_max + 2 s, s = sqrt(4 pi / npix) (more than any HEALPix
+     * pixel's centre-to-boundary distance). */
+    const double om = 4.0 * M_PI / (double)p.hs.npix;
+    const double rc = std::min(M_PI, p.rho_max + 2.0 * std::sqrt(om));
+    pl.bound = (long)(2.0 * M_PI * (1.0 - std::cos(rc)) / om) + 1;
+    pl.bound = std::min<long>(pl.bound, p.hs.npix);
+    /* per sample: its records and its offsets per band */
+    const double per_sample = (double)sizeof(BeamRec) * pl.bound + 8.0 * pl.nband;
+    double chunk = (double)pool_bytes / per_sample;
+    chunk = std::max(chunk, (double)kBlock);
+    chunk = std::min(chunk, (double)((1L << (32 - kRecRowBits)) - 1));
+    chunk = std::min(chunk, (double)(0x7fffffffL / pl.bound));   /* 32-bit record positions */
+    pl.chunk = std::max<long>(1, std::min<long>((long)chunk, n));
+    /* parts per band: about eight blocks per CU (one fits a CU at a time),
+     * so that the last round of blocks is a small part of the launch */
+    pl.nsplit = (int)std::max<long>(1, std::min<long>(64, (8L * std::max(ncu, 1) + pl.nband - 1) /
+                                                               pl.nband));
+    pl.nrec = (size_t)pl.chunk * (size_t)pl.bound;
+    pl.nidx = (size_t)pl.chunk * pl.nband + pl.nband + 1;
+    pl.npart = (size_t)pl.nsplit * 6 * (size_t)p.npix_beam;
+    return true;
+}
+
+hipError_t launch_adjbeam_records(const PairParams &p0, const RecPlan &pl, void *recs, long *idx,
+                                  float *vmax, double *part, hipStream_t st)
+{
+    if (p0.s_end <= p0.s_begin || p0.npix_beam <= 0)
+        return hipSuccess;
+    RecParams R;
+    R.rows = pl.rows;
+    R.nband = pl.nband;
+    R.inv_rows = 1.0f / (float)pl.rows;
+    R.brec = (BeamRec *)recs;
+    R.nrec_cap = (long)pl.nrec;
+    R.bcnt = idx;
+    R.bbase = idx + (size_t)pl.chunk * pl.nband;
+    R.chunk = pl.chunk;
+    R.vmax = vmax;
+    R.part = part;
+    R.nsplit = pl.nsplit;
+    const size_t bytes = 6 * sizeof(unsigned long long) * (size_t)pl.rows;
+    static std::atomic<size_t> configured[kDevs];
+    hipError_t e = ensure_dyn_lds((const void *)k_beamrec_sum, bytes, configured);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_fill_f64_n, dim3(gs_blocks((long)pl.npart)), dim3(256), 0, st, part,
+                       (long)pl.npart);
+    hipLaunchKernelGGL(k_fill_f64_n, dim3(1), dim3(64), 0, st, (double *)vmax, 1L);
+    hipLaunchKernelGGL(k_rec_vmax, dim3(1024), dim3(256), 0, st, p0, vmax);
+    for (long s0 = p0.s_begin; s0 < p0.s_end; s0 += pl.chunk) {
+        PairParams p = p0;
+        p.s_begin = s0;
+        p.s_end = std::min(s0 + pl.chunk, p0.s_end);
+        const long ns = p.s_end - p.s_begin;
+        const dim3 g1((unsigned)((ns + kWavesPerBlock - 1) / kWavesPerBlock));
+        hipLaunchKernelGGL(k_beamrec<false>, g1, dim3(kBlock), 0, st, p, R);
+        hipLaunchKernelGGL(k_band_scan, dim3((unsigned)pl.nband), dim3(kScanBlock), 0, st, R, ns);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(kScanBlock), 0, st, R.bbase,
+                           (long)pl.nband);
+        hipLaunchKernelGGL(k_beamrec<true>, g1, dim3(kBlock), 0, st, p, R);
+        hipLaunchKernelGGL(k_beamrec_sum, dim3((unsigned)(pl.nband * pl.nsplit)), dim3(kAdjBlock),
+                           bytes, st, p, R);
+    }
+    hipLaunchKernelGGL(k_beamrec_reduce, dim3(gs_blocks(6 * (long)p0.npix_beam)), dim3(256), 0, st,
+                       part, pl.nsplit, (long)p0.npix_beam, p0.gbeam);
     return hipGetLastError();
 }
 

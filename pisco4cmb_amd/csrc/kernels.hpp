@@ -148,6 +148,39 @@ hipError_t launch_fwd_sub(int sub, const PairParams &p, hipStream_t st, bool vte
 hipError_t launch_adjbeam_bands(const PairParams &p, int chunk, int rows, int nblocks, int ncu,
                                 int lanes, hipStream_t st);
 int adjbeam_band_rows();
+
+/* Beam adjoint through visit records: each visit mapped twice (count, then
+ * write its record in the order of its band of beam rows), bands summed in
+ * LDS; p.gbeam accumulates in FP64, p.sky4 holds the packed sky. */
+constexpr int kRecMaxBands = 512;    /* more bands: the banded kernels */
+struct BeamRec;
+struct RecParams {
+    int rows, nband;
+    float inv_rows;
+    BeamRec *brec;        /* records in band order, at most chunk x bound */
+    long nrec_cap;        /* records brec holds (below 2^31) */
+    long *bcnt;           /* [nband][chunk] records per (band, sample), then their offsets */
+    long chunk;           /* samples per chunk (the row length of bcnt) */
+    long *bbase;          /* [nband + 1] records per band, then the bands' starts */
+    const float *vmax;    /* device: max |g|, max over sky pixels of max(|I|, |Q| + |U|) */
+    double *part;         /* [nsplit][npix_beam][6] partial sums */
+    int nsplit;
+};
+struct RecPlan {
+    int rows, nband, nsplit;
+    long chunk;           /* samples per chunk */
+    long bound;           /* visits per sample, at most */
+    size_t nrec;          /* records (16 bytes) */
+    size_t nidx;          /* longs: bcnt, bbase */
+    size_t npart;         /* doubles of the partial sums */
+};
+/* rows: rows per band (0: as many as the band sums' LDS holds); pool_bytes:
+ * the memory budget of the records and offsets, which sets the samples per
+ * chunk.  False when the beam needs more than kRecMaxBands bands. */
+bool adjbeam_rec_plan(const PairParams &p, int rows, size_t pool_bytes, int ncu, RecPlan &pl);
+/* vmax: two device floats of scratch. */
+hipError_t launch_adjbeam_records(const PairParams &p, const RecPlan &pl, void *recs, long *idx,
+                                  float *vmax, double *part, hipStream_t st);
 /* Sky adjoint with LDS-privatised windows over chunks of `chunk` samples;
  * `cap` window pixels and `rw` rings per chunk; nblocks persistent blocks. */
 /* fx_scale (device, or NULL): deterministic mode with the scale of

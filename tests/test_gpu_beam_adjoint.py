@@ -1,12 +1,17 @@
-"""The beam adjoint (TOD -> beam maps) in LDS bands of beam rows (tuning key
-adjbeam_bands = 1, the default; 8 or 64 lanes per sample, adjbeam_lanes)
-against the oracle's FP64 beam adjoint and against the global-atomic form
-(adjbeam_bands = 0), relative RMS 1e-6 per column: the headline beam shape
-(81 bands), a cap reaching the equatorial rings of a coarse beam map (bands
-of 300 rows: whole and partial rings), small and large work items, samples
-near the poles; and the dot product with the forward.  The headline shape and
-the dot product run on co-polar and cross-polar beams (the gradient's
-columns do not depend on the beam values, the dot product does)."""
+"""The beam adjoint (TOD -> beam maps) against the oracle's FP64 beam adjoint,
+relative RMS 1e-6 per column, in each of its forms: visit records written
+in band order and summed band by band in LDS as 64-bit fixed-point integers
+(adjbeam_records = 1, the default; also with a 1-MiB pool, so that the scan
+runs in chunks of 256 samples), the annulus walks over LDS bands
+(adjbeam_records = 0; 8 or 64 lanes per sample, small and large work items)
+and global atomics (adjbeam_bands = 0).  Shapes: the headline beam (113
+bands of records, 81 annulus bands), a cap reaching the equatorial rings of
+a coarse beam map (bands of 300 rows: whole and partial rings), samples near
+the poles, small discs, beams of one ring and a last band of one row; the
+record form's bits repeat from call to call; and the dot product with the
+forward.  The headline shape and the dot
+product run on co-polar and cross-polar beams (the gradient's columns do not
+depend on the beam values, the dot product does)."""
 import math
 
 import numpy as np
@@ -33,6 +38,19 @@ def conv_for(ns, sky, beam, ra, dec, pa):
     return conv
 
 
+DEFAULTS = {"adjbeam_bands": 1, "adjbeam_records": 1, "adjbeam_pool_mb": 8192,
+            "adjbeam_chunk": 1024, "adjbeam_lanes": 8}
+
+
+def forms(chunks):
+    """Tuning of each form of the beam adjoint (keys not named: defaults)."""
+    out = [{}, {"adjbeam_pool_mb": 1}]
+    out += [{"adjbeam_records": 0, "adjbeam_chunk": c} for c in chunks]
+    out += [{"adjbeam_records": 0, "adjbeam_chunk": chunks[0], "adjbeam_lanes": 64},
+            {"adjbeam_bands": 0}]
+    return out
+
+
 def check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, chunks=(1024,), rows=0):
     rng = np.random.default_rng(81)
     ga, gb = (rng.normal(size=ra.size).astype(np.float32) for _ in range(2))
@@ -40,18 +58,16 @@ def check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, chunks=(1024,), row
     G = oracle.adjoint_beam(ns, *sky[:3], beam.nside, beams, ra, dec, pa, ga, gb)
     conv = conv_for(ns, sky, beam, ra, dec, pa)
     conv.set_tuning("adjbeam_rows", rows)
-    res = {}
-    for bands, chunk, lanes in ([(1, c, 8) for c in chunks] + [(1, chunks[0], 64)] +
-                                [(0, 1024, 8)]):
-        conv.set_tuning("adjbeam_bands", bands)
-        conv.set_tuning("adjbeam_chunk", chunk)
-        conv.set_tuning("adjbeam_lanes", lanes)
+    res = []
+    for form in forms(chunks):
+        for k, v in dict(DEFAULTS, **form).items():
+            conv.set_tuning(k, v)
         oa, ob = conv.exec_adjoint_beam(ga, gb)
         for k, (got, col) in enumerate(((oa[0], 0), (oa[1], 1), (oa[2], 2),
                                         (ob[0], 3), (ob[1], 4), (ob[2], 5))):
-            assert rel_rms(got, G[:, col]) < 1e-6, (bands, chunk, lanes, k)
+            assert rel_rms(got, G[:, col]) < 1e-6, (form, k)
         assert np.all(oa[3] == 0) and np.all(ob[3] == 0)
-        res[(bands, chunk, lanes)] = (oa, ob)
+        res.append((form, oa, ob))
     return conv, ga, gb, res
 
 
@@ -61,7 +77,14 @@ def test_headline_beam_shape(oracle, kind):
     sky = inputs.iid_sky(ns, seed=82)
     beam = inputs.make_polbeam(kind, 4096, 5.0, 0.33, 0.05, 0.01, nside_sky=ns)
     ra, dec, pa = inputs.ces_scan(400, t0=1300.0)
-    check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, chunks=(1024, 7))
+    conv, ga, gb, res = check_against_oracle(oracle, ns, sky, beam, ra, dec, pa,
+                                             chunks=(1024, 7))
+    # the record form sums integers: the same bits on every call
+    for k, v in DEFAULTS.items():
+        conv.set_tuning(k, v)
+    again = conv.exec_adjoint_beam(ga, gb)
+    for x, y in zip(res[0][1] + res[0][2], again[0] + again[1]):
+        assert np.array_equal(x.view(np.int32), y.view(np.int32))
     check_against_oracle(oracle, ns, sky, beam, ra, dec, pa, rows=1000)
 
 
@@ -80,6 +103,15 @@ def test_coarse_beam_equatorial_rings_and_poles(oracle):
     pa = rng.uniform(-math.pi, math.pi, n).astype(np.float32)
     check_against_oracle(oracle, ns, sky, b, ra, dec, pa, chunks=(1024, 5), rows=300)
     check_against_oracle(oracle, ns, sky, b, ra, dec, pa)
+
+
+def test_small_discs(oracle):
+    """Configuration 2's beam shape (about 134 sky pixels per disc)."""
+    ns = 256
+    sky = inputs.iid_sky(ns, seed=91)
+    beam = inputs.crosspolar_polbeam(2048, 1.5, 0.5, 0.05, 0.01, nside_sky=ns)
+    ra, dec, pa = inputs.ces_scan(3000, t0=91.0)
+    check_against_oracle(oracle, ns, sky, beam, ra, dec, pa)
 
 
 @pytest.mark.parametrize("kind", ["copolar", "crosspolar"])
@@ -120,7 +152,8 @@ def test_small_beams_and_one_cotangent(oracle, npix, rows):
     G = oracle.adjoint_beam(ns, *sky[:3], 64, beams, ra, dec, pa, ga, None)
     conv = conv_for(ns, sky, b, ra, dec, pa)
     conv.set_tuning("adjbeam_rows", rows)
-    for lanes in (8, 64):
+    for records, lanes in ((1, 8), (0, 8), (0, 64)):
+        conv.set_tuning("adjbeam_records", records)
         conv.set_tuning("adjbeam_lanes", lanes)
         oa, ob = conv.exec_adjoint_beam(ga, None)
         for k in range(3):
@@ -138,7 +171,8 @@ sky = inputs.iid_sky(ns, seed=89)
 ra, dec, pa = inputs.ces_scan(600, t0=40.0)
 g = np.random.default_rng(90).normal(size=(2, ra.size)).astype(np.float32)
 out = {}
-for npb_deg, nb, rows in ((3.0, 256, 300), (3.0, 1024, 0)):
+for records, npb_deg, nb, rows in ((0, 3.0, 256, 300), (0, 3.0, 1024, 0), (1, 3.0, 256, 300),
+                                   (1, 3.0, 1024, 0)):
     beam = inputs.gaussian_polbeam(nb, npb_deg, 1.0, 0.05, 0.01, nside_sky=ns)
     conv = pypisco.GPUConvolver(pypisco.default_run_config())
     conv.configure_execution()
@@ -146,6 +180,7 @@ for npb_deg, nb, rows in ((3.0, 256, 300), (3.0, 1024, 0)):
     conv.update_beam(beam)
     conv.update_scan(pypisco.Scan(ra.size, ra, dec, pa))
     conv.set_tuning("adjbeam_rows", rows)
+    conv.set_tuning("adjbeam_records", records)
     banded = conv.exec_adjoint_beam(g[0], g[1])
     conv.set_tuning("adjbeam_bands", 0)
     atomic = conv.exec_adjoint_beam(g[0], g[1])
@@ -162,7 +197,8 @@ def test_small_band_rows_first_in_a_fresh_process():
     """A process whose first banded beam adjoint uses small bands (300 rows)
     and whose second uses the default bands of an 8,580-row beam: the second
     launch needs a larger dynamic-LDS limit than the first set, and must
-    raise it (both equal the global-atomic form)."""
+    raise it (both equal the global-atomic form).  The same for the record
+    path's band sums, after the annulus walks."""
     import os
     import subprocess
     import sys
