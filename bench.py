@@ -21,7 +21,7 @@ with |D_i| the exact disc sizes (device query_disc).  achieved = bytes of
 the dominant kernel / its mean duration (HIP events on the launch stream).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--samples N]
-                  [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker] [--c5]
+                  [--no-cpu] [--no-forms] [--no-batched] [--no-mapmaker] [--no-c5]
                   [--beam-adjoint-samples M] [--calib-copy-mib M]
 
 The beam adjoint (TOD -> beam maps) is timed over the headline scan (or the
@@ -49,6 +49,42 @@ HBM_PEAK = 8.0e12   # bytes/s, MI355X
 # HBM-side bytes per launch of the headline workload's kernels, measured with
 # rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy of known size)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_nside512_cap5_4e6.txt")
+# hardware counters (VALU and TA busy, waits, L2 hits) of the same kernels on a
+# 1e6-sample prefix, measured with rocprofv3 --pmc
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_kernels_nside512_cap5_1e6.txt")
+
+
+def measured_limiter(kernel):
+    """VALU busy, TA busy, wait fraction and L2 hit rate of `kernel` (its
+    largest grid) from the committed counter summary, and what they say
+    bounds it; None if the file or the kernel is absent."""
+    try:
+        lines = open(PMC_FILE).read().splitlines()
+    except OSError:
+        return None
+    best = None
+    for l in lines:
+        t = l[30:].split()
+        if len(t) != 9 or l[:30].strip() != kernel:
+            continue
+        try:
+            grid, valu, ta, wait, hit = int(t[-9]), float(t[-7]), float(t[-6]), float(t[-5]), \
+                float(t[-4])
+        except ValueError:
+            continue
+        if best is None or grid > best[0]:
+            best = (grid, valu, ta, wait, hit)
+    if best is None:
+        return None
+    grid, valu, ta, wait, hit = best
+    if valu >= 0.95:
+        lim = "VALU issue"
+    elif ta >= 0.85:
+        lim = "texture addressing (TA) of the gathers"
+    else:
+        lim = "latency: VALU %.0f%% busy, waits %.0f%% of wave cycles" % (100 * valu, 100 * wait)
+    return {"valu_busy": valu, "ta_busy": ta, "wait_frac": wait, "l2_hit_rate": hit,
+            "limiter": lim, "counters": os.path.relpath(PMC_FILE, ROOT)}
 
 
 def measured_traffic(samples, kernel, what="traffic_bytes_per_launch"):
@@ -78,8 +114,8 @@ def parse():
                    help="skip the plain drop-in forward of the NSIDE 256 workload")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched 64-detector workload")
-    p.add_argument("--c5", action="store_true",
-                   help="also time the large configuration (NSIDE 2048, 256 detectors)")
+    p.add_argument("--no-c5", action="store_true",
+                   help="skip the large configuration (NSIDE 2048, 256 detectors)")
     p.add_argument("--beam-adjoint-samples", type=int, default=-1,
                    help="prefix of the headline scan on which the beam adjoint is timed "
                         "(-1: the whole scan; 0: skip)")
@@ -228,7 +264,10 @@ def main():
         beam_adj = {"samples": nbp, "s": t_adjb, "tod_samples_per_s": k * nbp / t_adjb,
                     "hbm_frac": b_adjb / t_adjb / HBM_PEAK,
                     "form": "visit records in band order (two mapping passes), band sums in "
-                            "LDS as 64-bit fixed point"}
+                            "LDS as 64-bit fixed point",
+                    "measured_limiters": {k: measured_limiter(k) for k in
+                                          ("k_beamrec<false>", "k_beamrec<true>",
+                                           "k_beamrec_sum")}}
         # the annulus-band form, on a prefix
         nba = min(nbp, 250_000)
         check(lib.pisco_set_tuning(ctx, b"adjbeam_records", 0))
@@ -266,11 +305,16 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(n, k_dom),
                      "l2_hit_rate": measured_traffic(n, k_dom, "l2_hit_rate"),
-                     "kernel": k_dom, "bytes_per_launch": b_dom, "mean_launch_s": t_dom},
+                     "kernel": k_dom, "bytes_per_launch": b_dom, "mean_launch_s": t_dom,
+                     "measured_limiter": measured_limiter(
+                         "k_adjmap_lds<0, false, 64>" if k_dom == "k_adjmap_lds" else k_dom),
+                     "note": "frac prices the gather model against HBM; the counters show "
+                             "what bounds the kernel (measured_limiter)"},
         "fwd_adj": {"forward_s": t_fwd, "adjoint_s": t_adj,
                     "adjoint_deterministic_s": t_adj_det,
                     "frac_hbm_fwd_plus_adj": (b_fwd + b_adj) / (t_fwd + t_adj) / HBM_PEAK,
                     "forward_tod_samples_per_s": k * n / t_fwd,
+                    "forward_measured_limiter": measured_limiter("k_pair<0>"),
                     "wall_s_timed": wall, "setup_s": setup_s},
     }
     if beam_adj:
@@ -282,7 +326,7 @@ def main():
         out["batched_64det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky)
         out["batched_5deg_16det"] = batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky,
                                                      npairs=8, five_deg=True)
-    if args.c5:
+    if not args.no_c5:
         out["large_256det"] = large_workload(dev)
     if not args.no_mapmaker:
         out["map_maker_64det"] = map_maker_workload(dev, nside_sky, with_cpu=not args.no_cpu)
@@ -488,7 +532,9 @@ def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=
             "plain_fwd_adj": {"det_samples_per_s": det / (t_pf + t_pa), "s": t_pf + t_pa,
                               "hbm_frac": b_fa / (t_pf + t_pa) / HBM_PEAK},
             "batched_fwd_adj_vs_plain_forward": t_pf / t_step,
-            "batched_fwd_adj_vs_plain_fwd_adj": (t_pf + t_pa) / t_step}
+            "batched_fwd_adj_vs_plain_fwd_adj": (t_pf + t_pa) / t_step,
+            "measured_limiters": {k: measured_limiter(k) for k in
+                                  ("k_batch_fwd<4>", "k_adjmap_lds<4, false, 64>")}}
 
 
 def large_workload(dev, n=100_000, per_family=32, reps=1):

@@ -439,7 +439,9 @@ int pisco_pix2ang(int nside, long pix, double *theta, double *phi);
  * for band `band` (beam rows [band*rows, min(npix_beam, (band+1)*rows)) of
  * a beam map of NSIDE nside_beam and npix_beam pixels) around the pointing
  * (ra, dec): the annulus runs of every sky ring, as the kernels form them
- * (cover != 0: the float-span form the kernels use; 0: exact spans).  Writes
+ * (cover != 0: the float-span form the kernels use, with an IEEE quotient
+ * where the kernels use the hardware reciprocal estimate, within 1 ulp of
+ * it and inside the cover's 4e-6 rad margin; 0: exact spans).  Writes
  * at most cap pixel indices to pix, in walk order, and returns their
  * number; -1 on bad arguments, -2 if more than cap.  Host only.  No
  * counterpart in the reference (its beam maps have no gradient); it exposes
