@@ -4,7 +4,9 @@ integers, so the output bits do not depend on the schedule.  Checked: the
 same bits for different launch shapes (persistent blocks, samples per
 window) and repeated calls, for the single-pair, batched and per-pair-pointing
 adjoints; within relative RMS 1e-6 of the floating-point adjoint and of the
-CPU oracle; the dot-product identity; refused with beam interpolation."""
+CPU oracle; the dot-product identity; refused with beam interpolation; the
+default (-1) gives the integer sums' bits for single pairs, and with beam
+interpolation falls back to float sums instead of refusing."""
 import math
 
 import numpy as np
@@ -33,7 +35,7 @@ def rel_rms(a, b):
 def ctx():
     c = torch_ops._ctx(torch.device("cuda:0"))
     yield c
-    for k, v in ((b"adjmap_deterministic", 0), (b"adjmap_blocks", 0), (b"adjmap_chunk", -1)):
+    for k, v in ((b"adjmap_deterministic", -1), (b"adjmap_blocks", 0), (b"adjmap_chunk", -1)):
         check(lib.pisco_set_tuning(c, k, v))
 
 
@@ -56,14 +58,17 @@ def test_single_pair_bits_independent_of_launch_shape(ctx, oracle, kind):
     rng = np.random.default_rng(41)
     ga, gb = (rng.normal(size=ra.size).astype(np.float32) for _ in range(2))
     args = (beam8, nb, t(ra), t(dec), t(pa), t(ga), t(gb), ns)
+    check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 0))
     ref = adjoint(ctx, *args)   # floating-point mode
+    check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", -1))
+    default = adjoint(ctx, *args)   # the default: integer sums for a single pair
     check(lib.pisco_set_tuning(ctx, b"adjmap_deterministic", 1))
     runs = []
     for blocks, chunk in ((0, 16), (97, 8), (300, 32), (0, 16)):
         check(lib.pisco_set_tuning(ctx, b"adjmap_blocks", blocks))
         check(lib.pisco_set_tuning(ctx, b"adjmap_chunk", chunk))
         runs.append(adjoint(ctx, *args))
-    for r in runs[1:]:
+    for r in runs[1:] + [default]:
         for a, b in zip(r, runs[0]):
             assert np.array_equal(a.view(np.int32), b.view(np.int32))
     for a, b in zip(runs[0], ref):
@@ -100,6 +105,7 @@ def test_batched_bits_independent_of_launch_shape(per_pair):
         ra, dec, pa = (np.stack([x + np.float32(0.01 * k) for k in range(5)]) for x in
                        (ra, dec, pa))
     g = torch.randn((5, 2, 3000), device="cuda")
+    batch.set_tuning("adjmap_deterministic", 0)
     ref = [x.cpu().numpy() for x in batch.adjoint_map(g, t(ra), t(dec), t(pa), ns)]
     batch.set_tuning("adjmap_deterministic", 1)
     runs = []
@@ -128,3 +134,27 @@ def test_refused_with_interpolation(ctx):
             adjoint(ctx, beam8, nb, ra, dec, pa, g, g, ns)
     finally:
         check(lib.pisco_set_tuning(ctx, b"beam_interpolation", 0))
+
+
+def test_default_with_interpolation_uses_float_sums(ctx, oracle):
+    """adjmap_deterministic -1 (the default) with beam interpolation: the
+    float sums, against the oracle's interpolation mode."""
+    ns, nb = 32, 256
+    beam = inputs.gaussian_polbeam(nb, 4.0, 1.5, nside_sky=ns)
+    beam8 = torch_ops.pack_beam(t(np.stack(beam.aBeams)), t(np.stack(beam.bBeams)))
+    ra, dec, pa = inputs.raster_scan(400)
+    rng = np.random.default_rng(43)
+    ga, gb = (rng.normal(size=ra.size).astype(np.float32) for _ in range(2))
+    check(lib.pisco_set_tuning(ctx, b"beam_interpolation", 1))
+    try:
+        got = adjoint(ctx, beam8, nb, t(ra), t(dec), t(pa), t(ga), t(gb), ns)
+    finally:
+        check(lib.pisco_set_tuning(ctx, b"beam_interpolation", 0))
+    oracle.set_beam_interpolation(True)
+    try:
+        ref = oracle.adjoint_map(ns, nb, list(beam.aBeams) + list(beam.bBeams), ra, dec, pa, ga,
+                                 gb)
+    finally:
+        oracle.set_beam_interpolation(False)
+    for a, b in zip(got, ref):
+        assert rel_rms(a, b) < 1e-6
