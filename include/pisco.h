@@ -161,11 +161,11 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * "adjmap_deterministic" (sky adjoint sums: 1 rounds every contribution
  * once to a multiple of 2^-k and sums 64-bit integers, so the output bits
  * do not depend on the order of the additions, the launch shape or the
- * block count; 0 float sums; -1, the default, integer sums wherever one
- * pair runs per wave -- single-pair and pair-major launches with
- * nearest-pixel beams, where they also measured faster -- and float sums
- * in the grouped batched kernels, whose pairs share geometry; 1 is refused
- * with beam interpolation),
+ * block count (batched launches: for a given grouping of the pairs, each
+ * visit's sum over a group formed in pair order); 0 float sums; -1, the
+ * default, integer sums for nearest-pixel beams, where they also measured
+ * faster, and float sums with beam interpolation; 1 is refused with beam
+ * interpolation),
  * "v_term" (host-buffer calls: 1 adds the reference GPU kernel's V A3 and
  * V B3 (cudaConv2.cu:192,204,214,217-218) to pisco_exec_convolution's TOD,
  * with update_sky's V (zero if it was NULL), and fills the a[3]/b[3]

@@ -175,8 +175,8 @@ struct pisco_ctx {
     int batch_adj_group = 4;      /* pairs per wave in batched sky adjoints */
     int beam_interp = 0;          /* 1: beam interpolated from get_interpol neighbours */
     int adjmap_det = -1;          /* sky adjoint sums: 1 integer (deterministic), 0 float,
-                                     -1 integer where one pair runs per wave (single-pair
-                                     and pair-major launches, nearest-pixel beams) */
+                                     -1 integer for nearest-pixel beams, float with beam
+                                     interpolation */
     int forward_lanes = 0;        /* lanes per sample of the forward: 64, 32, 16, 8; 0 = auto */
     int adjmap_lanes = 0;         /* lanes per sample of the sky adjoint: 64, 16, 8; 0 = auto */
     int batch_pair_major = -1;    /* batched launches pair by pair: 1 yes, 0 no, -1 auto */
@@ -208,11 +208,11 @@ struct pisco_ctx {
 
 namespace {
 
-/* Integer (deterministic) sums for sky adjoints that run one pair per wave:
- * the default -1 takes them wherever the beam is nearest-pixel (they also
- * measured faster there: 50.0 against 51.0 ms per 1e6 samples at the
- * 5-degree, NSIDE 512 shape, 2.81 against 3.19 ms at 1.5 degrees, NSIDE
- * 256). */
+/* Integer (deterministic) sums in the sky adjoint: the default -1 takes them
+ * wherever the beam is nearest-pixel.  They also measured faster: 50.0
+ * against 51.0 ms per 1e6 samples at the 5-degree, NSIDE 512 shape, 2.81
+ * against 3.19 ms at 1.5 degrees, NSIDE 256, and 56.9 against 60.0 ms for
+ * the batched 32-pair configuration-4 adjoint with groups of four pairs. */
 bool det_single(const pisco_ctx *c)
 {
     return c->adjmap_det > 0 || (c->adjmap_det < 0 && !c->beam_interp);
@@ -1706,10 +1706,8 @@ int pisco_dev_batched_adjoint_map(pisco_ctx *c, void *stream, const pisco_batch 
     bp.gU = c->acc.p + 2 * npix;
     const int nb = c->adjmap_blocks;   /* 0: one block per window */
     const int chunk = c->adjmap_chunk > 0 ? c->adjmap_chunk : 16;
-    const bool det = c->adjmap_det > 0;
-    /* the deterministic mode takes one pair per wave (integer windows); the
-     * default keeps the float sums here, so that pairs share geometry */
-    const int grp = det ? 1 : c->batch_adj_group;
+    const bool det = det_single(c);
+    const int grp = c->batch_adj_group;
     /* the window capacity that fits next to the group's per-wave tables */
     const int cap = fit_cap(c->adjmap_cap, c->adjmap_rw, grp, det);
     const double *scale = nullptr;

@@ -1244,10 +1244,11 @@ struct AdjWaveTable<1> {
 
 /* BG: 0 one pair; 1 a batch, one wave per (pair, sample); 2, 4 a batch, one
  * wave per (group of BG pairs, sample) sharing each visit's geometry.
- * DET: deterministic mode (BG 0 and 1): every contribution is rounded once
- * to a multiple of 1 / scale and all sums are 64-bit integer sums, which do
- * not depend on the order of the additions; the result is the same bits for
- * any schedule and launch shape. */
+ * DET: deterministic mode: every contribution (with BG >= 2, each visit's
+ * sum over the group's pairs, formed in pair order) is rounded once to a
+ * multiple of 1 / scale and all sums are 64-bit integer sums, which do not
+ * depend on the order of the additions; the result is the same bits for any
+ * schedule and launch shape (for a given grouping of the pairs). */
 template <int BG, bool DET = false, int SUB = kWaveSize>
 __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayout L)
 {
@@ -2848,8 +2849,6 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
     }
     const int bg = batch ? group : 0;
     const bool det = fx_scale != nullptr;
-    if (det && bg > 1)
-        return hipErrorInvalidValue;   /* deterministic mode: one pair per wave */
     L.fx_scale = fx_scale;
     const size_t bytes = adjmap_lds_bytes(cap, rw, bg, det);
     /* nblocks 0: one block per window.  Blocks start in index order, so the
@@ -2868,9 +2867,15 @@ hipError_t launch_adjmap_lds(const PairParams &p, int chunk, int cap, int rw, in
         default: return hipErrorInvalidValue;
         }
     }
-    if (det)
-        return bg == 0 ? launch_adjmap_bg<0, true>(p, L, bytes, g, st)
-                       : launch_adjmap_bg<1, true>(p, L, bytes, g, st);
+    if (det) {
+        switch (bg) {
+        case 0: return launch_adjmap_bg<0, true>(p, L, bytes, g, st);
+        case 1: return launch_adjmap_bg<1, true>(p, L, bytes, g, st);
+        case 2: return launch_adjmap_bg<2, true>(p, L, bytes, g, st);
+        case 4: return launch_adjmap_bg<4, true>(p, L, bytes, g, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (bg) {
     case 0: return launch_adjmap_bg<0, false>(p, L, bytes, g, st);
     case 1: return launch_adjmap_bg<1, false>(p, L, bytes, g, st);
