@@ -527,19 +527,75 @@ __device__ inline int build_sub(const Hpx &hs, const Disc &d, int nrings, int ps
     return total;
 }
 
+/* Per-wave state of the forward's deferred visits in the several-samples-
+ * per-wave kernels: the queue (sky pixel, ring, lane group) and, per lane
+ * group, its sample's beam-centre constants and the FP64 sums of its
+ * deferred visits' TOD. */
+template <int SUB>
+struct SubDefer {
+    static constexpr int G = kWaveSize / SUB;
+    int qpix[2 * kWaveSize];
+    int qring[2 * kWaveSize];
+    int qgrp[2 * kWaveSize];
+    double ra[G], psi[G], cdc[G], sdc[G], c2psi[G], s2psi[G];
+    double acc[G][2];
+};
+
+/* Maps the queued visits base .. base+cnt-1 (cnt <= 64, one per lane) with
+ * the FP64-alpha form, each with its own lane group's sample, and adds their
+ * TOD to that group's sums (whole wave). */
+template <int SUB, bool VT>
+__device__ inline void drain_sub(const PairParams &p, SubDefer<SUB> &q, int base, int cnt, int lane)
+{
+    wave_sync();
+    if (lane < cnt) {
+        const int pix = q.qpix[base + lane], r = q.qring[base + lane], g = q.qgrp[base + lane];
+        Centre c;
+        c.ra = q.ra[g];
+        c.psi = q.psi[g];
+        c.cdc = q.cdc[g];
+        c.sdc = q.sdc[g];
+        c.dec = 0.0;   /* not read by the mapping */
+        const RingRec rr = p.rings[r];
+        Pending e;
+        e.bp = beam_pixel_f64(p.hb, p.fc64, c, q.c2psi[g], q.s2psi[g],
+                              ring_phi(p.hs, rr, r, pix), rr.cdp, rr.sdp, e.c2, e.s2);
+        if ((unsigned)e.bp < (unsigned)p.npix_beam) {
+            const float *row = p.beam + 8 * (long)e.bp;
+            e.v0 = *(const float4 *)row;
+            e.v1 = *(const float2 *)(row + 4);
+            if constexpr (VT)
+                e.v2 = *(const float2 *)(row + 6);
+            const float4 sv = p.sky4[pix];
+            e.sI = sv.x;
+            e.sQ = sv.y;
+            e.sU = sv.z;
+            e.sV = sv.w;
+            float ta, tb;
+            visit_tod<VT>(e, ta, tb);
+            atomicAdd(&q.acc[g][0], (double)ta);
+            atomicAdd(&q.acc[g][1], (double)tb);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);   /* as in drain_exact */
+    wave_sync();
+}
+
 /* The walk of one ring pass of a lane group: its `total` visits, SUB at a
  * time, for wtot (the wave's largest total) iterations' worth, with the
  * software pipelining of walk_visits (next ring record prefetched; a visit's
  * loads consumed one iteration later) and the certified FP64 mapping.  In the
- * forward (LOAD_SKY) every load of the loop is issued unconditionally; the
- * adjoints load the beam row of live visits only.  A lane without a visit
- * computes the first visit of its run (or, in a group with no visit, pixel 0
- * of ring 1) and discards it.  use(q) is called once per visit whose beam
- * pixel lies inside the beam. */
+ * forward (LOAD_SKY) every load of the loop is issued unconditionally and the
+ * mapping takes its alpha from a float atan2; visits whose floors that
+ * cannot certify go to the wave's queue (dq, qn) and are drained 64 at a
+ * time.  The adjoints load the beam row of live visits only.  A lane without
+ * a visit computes the first visit of its run (or, in a group with no visit,
+ * pixel 0 of ring 1) and discards it.  use(q) is called once per visit whose
+ * beam pixel lies inside the beam and is not deferred. */
 template <int SUB, bool LOAD_SKY, typename Use, bool VT = false>
 __device__ inline void walk_sub(const PairParams &p, const SubTable<SUB> &t, int total,
                                 int wtot, int sl, const Centre &c, double c2psi, double s2psi,
-                                Use &use)
+                                Use &use, SubDefer<SUB> *dq = nullptr, int *qn = nullptr)
 {
     if (wtot <= 0)
         return;
@@ -564,11 +620,29 @@ __device__ inline void walk_sub(const PairParams &p, const SubTable<SUB> &t, int
                 s++;
         rr = p.rings[any ? t.ring[s] : 1];
         float c2, s2;
-        int bp = beam_pixel_f64(p.hb, p.fc64, c, c2psi, s2psi, ring_phi(p.hs, cur, r, pix),
-                                cur.cdp, cur.sdp, c2, s2);
+        int bp = beam_pixel_f64<LOAD_SKY>(p.hb, p.fc64, c, c2psi, s2psi,
+                                          ring_phi(p.hs, cur, r, pix), cur.cdp, cur.sdp, c2, s2);
         bp = act ? bp : -1;
+        if constexpr (LOAD_SKY) {   /* float alpha: uncertain floors to the queue */
+            const unsigned long long m = __ballot(bp == kNeedExact);
+            if (m) {
+                if (bp == kNeedExact) {
+                    const int pos = *qn + lanes_below(m);
+                    dq->qpix[pos] = pix;
+                    dq->qring[pos] = r;
+                    dq->qgrp[pos] = (int)(__lane_id() / SUB);
+                }
+                *qn += __popcll(m);
+            }
+        }
         if (q.live)
             use(q);
+        if constexpr (LOAD_SKY) {
+            if (*qn >= kWaveSize) {
+                drain_sub<SUB, VT>(p, *dq, *qn - kWaveSize, kWaveSize, (int)__lane_id());
+                *qn -= kWaveSize;
+            }
+        }
         q.live = (unsigned)bp < (unsigned)p.npix_beam;
         q.pix = pix;
         q.r = r;
@@ -614,10 +688,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 {
     constexpr int G = kWaveSize / SUB;
     __shared__ SubTable<SUB> tables[kWavesPerBlock * G];
+    __shared__ SubDefer<SUB> defers[kWavesPerBlock];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
     const int lane = threadIdx.x % kWaveSize;
     const int grp = lane / SUB, sl = lane % SUB;
     SubTable<SUB> &t = tables[wave * G + grp];
+    SubDefer<SUB> &dq = defers[wave];
     const long lb = xcd_remap(blockIdx.x, gridDim.x);
     const long w0 = p.s_begin + (lb * kWavesPerBlock + wave) * G;   /* the wave's first sample */
     if (w0 >= p.s_end)
@@ -629,6 +705,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     Disc d;
     const int nrings = sub_setup(p, valid ? i : w0, valid, c, c2psi, s2psi, d);
     const int npass = (wave_max_uniform(nrings) + SUB - 1) / SUB;
+    if (sl == 0) {   /* the group's constants and sums of deferred visits */
+        dq.ra[grp] = c.ra;
+        dq.psi[grp] = c.psi;
+        dq.cdc[grp] = c.cdc;
+        dq.sdc[grp] = c.sdc;
+        dq.c2psi[grp] = c2psi;
+        dq.s2psi[grp] = s2psi;
+        dq.acc[grp][0] = 0.0;
+        dq.acc[grp][1] = 0.0;
+    }
+    wave_sync();
+    int qn = 0;   /* queued visits (wave-uniform) */
     double acc_a = 0.0, acc_b = 0.0;
     auto use = [&](const Pending &e) {
         float ta, tb;
@@ -639,15 +727,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     for (int ps = 0; ps < npass; ps++) {
         const int total = build_sub<SUB>(p.hs, d, nrings, ps, sl, t);
         walk_sub<SUB, true, decltype(use), VT>(p, t, total, wave_max_uniform(total), sl, c, c2psi,
-                                               s2psi, use);
+                                               s2psi, use, &dq, &qn);
         wave_sync();
     }
+    if (qn > 0)
+        drain_sub<SUB, VT>(p, dq, 0, qn, lane);
 #pragma unroll
     for (int m = SUB / 2; m > 0; m >>= 1) {
         acc_a += __shfl_xor(acc_a, m, SUB);
         acc_b += __shfl_xor(acc_b, m, SUB);
     }
     if (valid && sl == 0) {
+        acc_a += dq.acc[grp][0];
+        acc_b += dq.acc[grp][1];
         if (p.tod_a)
             p.tod_a[i] = (float)acc_a;
         if (p.tod_b)
