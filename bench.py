@@ -538,7 +538,7 @@ def batched_workload(ctx, stream, dev, It, Qt, Ut, nside_sky, n=250_000, npairs=
             "batched_fwd_adj_vs_plain_forward": t_pf / t_step,
             "batched_fwd_adj_vs_plain_fwd_adj": (t_pf + t_pa) / t_step,
             "measured_limiters": {k: measured_limiter(k) for k in
-                                  ("k_batch_fwd<4>", "k_adjmap_lds<4, false, 64>")}}
+                                  ("k_batch_fwd<4>", "k_adjmap_lds<4, true, 64>")}}
 
 
 def large_workload(dev, n=100_000, per_family=32, reps=1):
