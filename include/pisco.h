@@ -153,7 +153,8 @@ int pisco_exec_adjoint_beam(pisco_ctx *ctx, const float *g_a, const float *g_b,
  * integers, which makes the result independent of the order of the
  * additions (beams of up to 512 bands); 0 walks each sample's annulus
  * once per band), "adjbeam_pool_mb" (memory of the records, which sets the
- * samples per chunk, MiB; default 8192),
+ * samples per chunk, MiB; default 8192, and at most a quarter of the free
+ * device memory),
  * "adjbeam_lanes" (annulus walks: 8, the default, lanes per sample
  * with 8 samples per wave, or 64),
  * "adjbeam_chunk" (samples per annulus work item, default 1024),

@@ -1168,8 +1168,15 @@ int pisco_dev_adjoint_beam(pisco_ctx *c, void *stream, int nside_sky, const floa
         p.sky4 = (const float4 *)c->sky4.p;
         RecPlan pl;
         hipError_t e;
-        if (c->adjbeam_records &&
-            adjbeam_rec_plan(p, c->adjbeam_rows, (size_t)c->adjbeam_pool_mb << 20, c->ncu, pl)) {
+        /* the records' memory: adjbeam_pool_mb, at most a quarter of the free
+         * device memory (beside what the context already holds for them) */
+        size_t pool = (size_t)c->adjbeam_pool_mb << 20;
+        size_t mfree = 0, mtotal = 0;
+        if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess) {
+            const size_t held = c->recs.cap * sizeof(float4) + c->recidx.cap * sizeof(long);
+            pool = std::min(pool, std::max<size_t>((mfree + held) / 4, (size_t)64 << 20));
+        }
+        if (c->adjbeam_records && adjbeam_rec_plan(p, c->adjbeam_rows, pool, c->ncu, pl)) {
             CHK(c->recs.ensure(pl.nrec));
             CHK(c->recidx.ensure(pl.nidx));
             CHK(c->recvmax.ensure(2));
