@@ -2129,9 +2129,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band(PairParams p, BeamBa
  *     within the band; the bands' starts;
  *  3. k_beamrec<true> maps the visits again and writes each one's record
  *     (row within the band | sample << kRecRowBits, I, X = Q c2 + U s2,
- *     Y = -Q s2 + U c2) at its place in band order;
- *  4. k_beamrec_sum: a block per (band, part of the band's records) sums its
- *     records, times the cotangents of each record's sample, into the band's
+ *     Y = -Q s2 + U c2) into its band, within the range of the samples of
+ *     its block;
+ *  4. k_beamrec_sum: a block per (band, part: the records of a share of the
+ *     chunk's samples) sums its records, times the cotangents of each
+ *     record's sample, into the band's
  *     rows in LDS as 64-bit fixed-point integers, then adds them to its own
  *     FP64 partial rows (a (band, part) slice belongs to one block in every
  *     chunk: no global atomics).
@@ -2146,6 +2148,7 @@ struct BeamRec {
 };
 constexpr int kRecRowBits = 13;
 constexpr int kRecUnroll = 4;
+constexpr int kRecWriteWaves = 8;   /* samples per block of the write pass (shared cursors) */
 constexpr int kScanBlock = 1024;
 
 /* In-place exclusive scan of x[0 .. n) (one block); x[n] receives the total. */
@@ -2307,22 +2310,43 @@ struct RecSinkT : RecSink<WRITE> {
 
 /* Steps 1 and 3: one wave per sample of the chunk [p.s_begin, p.s_end). */
 template <bool WRITE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(WRITE ? kRecWriteWaves * kWaveSize : kBlock)
+__attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_beamrec(PairParams p, RecParams R)
 {
-    __shared__ RunTable tables[kWavesPerBlock];
-    __shared__ unsigned curs[kWavesPerBlock][kRecMaxBands];
+    constexpr int NW = WRITE ? kRecWriteWaves : kWavesPerBlock;   /* waves (samples) per block */
+    __shared__ RunTable tables[NW];
+    /* counts: one row per wave (sample); write cursors: one row per block */
+    __shared__ unsigned curs[WRITE ? 1 : NW][kRecMaxBands];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
     const int lane = threadIdx.x % kWaveSize;
     RunTable &t = tables[wave];
-    unsigned *cur = curs[wave];
-    const long j = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    const long j0 = xcd_remap(blockIdx.x, gridDim.x) * NW;
+    const long j = j0 + wave;
     const long i = p.s_begin + j;
-    if (i >= p.s_end)
-        return;
-    for (int b = lane; b < R.nband; b += kWaveSize)
-        cur[b] = WRITE ? (unsigned)(R.bbase[b] + R.bcnt[b * R.chunk + j]) : 0u;
-    wave_sync();
+    unsigned *cur;
+    if constexpr (WRITE) {
+        /* The block's samples j0 .. j0+NW-1 own adjacent ranges in every
+         * band (the offsets are an exclusive scan over samples), so the
+         * block fills each band's range through one shared cursor: a
+         * 128-byte line of records is completed by the block's waves
+         * together, while it is still in L2, instead of by one wave over
+         * many iterations (partly written lines left L2 and doubled the
+         * pass's HBM writes). */
+        cur = curs[0];
+        for (int b = threadIdx.x; b < R.nband; b += NW * kWaveSize)
+            cur[b] = (unsigned)(R.bbase[b] + R.bcnt[b * R.chunk + j0]);
+        __syncthreads();
+        if (i >= p.s_end)
+            return;
+    } else {
+        if (i >= p.s_end)
+            return;
+        cur = curs[wave];
+        for (int b = lane; b < R.nband; b += kWaveSize)
+            cur[b] = 0u;
+        wave_sync();
+    }
     RecSinkT<WRITE> sink{{R, cur, (unsigned)j << kRecRowBits}};
     const float ga = unif(p.g_a ? p.g_a[i] : 0.f);
     const float gb = unif(p.g_b ? p.g_b[i] : 0.f);
@@ -2403,8 +2427,16 @@ __global__ __launch_bounds__(kAdjBlock) void k_beamrec_sum(PairParams p, RecPara
     const int lo = b * R.rows;
     const int nrow = min(p.npix_beam - lo, R.rows);
     const long r0 = R.bbase[b], cnt = R.bbase[b + 1] - r0;
-    const long per = (cnt + R.nsplit - 1) / R.nsplit;
-    const long k0 = std::min(cnt, part * per), j1 = std::min(cnt, (part + 1) * per);
+    /* the part's samples: a share of the chunk's, cut at multiples of the
+     * write pass's samples per block, so that the records of a part are the
+     * same set however the block's waves interleaved them */
+    const long ns = p.s_end - p.s_begin;
+    const long sa = (ns * part / R.nsplit) / kRecWriteWaves * kRecWriteWaves;
+    const long sb = part + 1 == R.nsplit
+                        ? ns
+                        : (ns * (part + 1) / R.nsplit) / kRecWriteWaves * kRecWriteWaves;
+    const long k0 = sa < ns ? R.bcnt[b * R.chunk + sa] : cnt;
+    const long j1 = sb < ns ? R.bcnt[b * R.chunk + sb] : cnt;
     if (k0 >= j1)
         return;   /* no records (uniform per block) */
     const double vmax = (double)R.vmax[0] * (double)R.vmax[1] * (1.0 + 1e-3);
@@ -3089,7 +3121,8 @@ hipError_t launch_adjbeam_records(const PairParams &p0, const RecPlan &pl, void 
         hipLaunchKernelGGL(k_band_scan, dim3((unsigned)pl.nband), dim3(kScanBlock), 0, st, R, ns);
         hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(kScanBlock), 0, st, R.bbase,
                            (long)pl.nband);
-        hipLaunchKernelGGL(k_beamrec<true>, g1, dim3(kBlock), 0, st, p, R);
+        const dim3 g3((unsigned)((ns + kRecWriteWaves - 1) / kRecWriteWaves));
+        hipLaunchKernelGGL(k_beamrec<true>, g3, dim3(kRecWriteWaves * kWaveSize), 0, st, p, R);
         hipLaunchKernelGGL(k_beamrec_sum, dim3((unsigned)(pl.nband * pl.nsplit)), dim3(kAdjBlock),
                            bytes, st, p, R);
     }
