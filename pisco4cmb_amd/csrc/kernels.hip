@@ -2447,21 +2447,42 @@ __global__ __launch_bounds__(kAdjBlock) void k_beamrec_sum(PairParams p, RecPara
     for (int k = tid; k < 6 * nrow; k += kAdjBlock)
         acc[k] = 0ull;
     __syncthreads();
+    /* an absent cotangent reads the other one and adds zeros (with neither
+     * there are no records) */
+    const bool has_a = p.g_a != nullptr, has_b = p.g_b != nullptr;
+    const float *gA = has_a ? p.g_a : p.g_b, *gB = has_b ? p.g_b : p.g_a;
+    if (gA == nullptr)
+        return;
+    /* records of the next batch are loaded while the current one is summed */
+    BeamRec nx[kRecUnroll];
+#pragma unroll
+    for (int u = 0; u < kRecUnroll; u++) {
+        const long k = k0 + u * kAdjBlock + tid;
+        nx[u] = R.brec[r0 + (k < j1 ? k : k0)];
+    }
     for (long kk = k0; kk < j1; kk += (long)kRecUnroll * kAdjBlock) {
         BeamRec r[kRecUnroll];
         float ga[kRecUnroll], gb[kRecUnroll];
 #pragma unroll
         for (int u = 0; u < kRecUnroll; u++) {
             const long k = kk + u * kAdjBlock + tid;
-            r[u] = R.brec[r0 + (k < j1 ? k : k0)];
+            r[u] = nx[u];
             if (k >= j1)
                 r[u].sI = r[u].X = r[u].Y = 0.f;   /* adds zeros */
         }
+        /* the cotangents first, then the prefetch: loads complete in order,
+         * so waiting for the cotangents does not wait for the prefetch */
 #pragma unroll
         for (int u = 0; u < kRecUnroll; u++) {
             const long i = p.s_begin + (long)(r[u].key >> kRecRowBits);
-            ga[u] = p.g_a ? p.g_a[i] : 0.f;
-            gb[u] = p.g_b ? p.g_b[i] : 0.f;
+            const float a = gA[i], bb = gB[i];   /* unconditional: no branch between loads */
+            ga[u] = has_a ? a : 0.f;
+            gb[u] = has_b ? bb : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kRecUnroll; u++) {
+            const long kn = kk + u * kAdjBlock + tid + (long)kRecUnroll * kAdjBlock;
+            nx[u] = R.brec[r0 + (kn < j1 ? kn : k0)];
         }
 #pragma unroll
         for (int u = 0; u < kRecUnroll; u++) {
