@@ -239,7 +239,7 @@ __device__ inline void drain_exact(const PairParams &p, RunTable &t, int base, i
  * float alpha) go to the queue (qn entries, wave-uniform); a full batch of 64
  * is drained at once.  use(q) is called once per visit whose beam pixel lies
  * inside the beam. */
-template <bool LOAD_SKY, bool MAP64, typename Use, bool VT = false>
+template <bool LOAD_SKY, bool MAP64, typename Use, bool VT = false, bool LOAD_BEAM = true>
 __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, int lane,
                                    const Centre &c, const SampleF &sf, const SampleD &sd,
                                    int &qn, Use &use)
@@ -332,11 +332,13 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
             q.sQ = sv.y;
             q.sU = sv.z;
             q.sV = sv.w;
-        } else if (q.live) {
-            /* adjoints: only live visits load (measured faster there) */
-            const float *row = p.beam + 8 * (long)bp;
-            q.v0 = *(const float4 *)row;
-            q.v1 = *(const float2 *)(row + 4);
+        } else if constexpr (LOAD_BEAM) {
+            if (q.live) {
+                /* adjoints: only live visits load (measured faster there) */
+                const float *row = p.beam + 8 * (long)bp;
+                q.v0 = *(const float4 *)row;
+                q.v1 = *(const float2 *)(row + 4);
+            }
         }
     }
     if (q.live)
@@ -2364,15 +2366,30 @@ void k_beamrec(PairParams p, RecParams R)
         Disc d;
         sample_setup(p, i, p.pa[i], c, sf, sd, d);
         int qn = 0;
-        for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
-            if (r0 + lane <= d.r_last)
-                rl[lane] = p.rings[r0 + lane];
-            const int total = build_runs(p.hs, d, r0, lane, t);   /* ends with a wave sync */
-            walk_records<WRITE>(p, t, rl, r0, total, lane, c, sd, qn, sink);
-            wave_sync();
+        if constexpr (WRITE) {
+            for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+                if (r0 + lane <= d.r_last)
+                    rl[lane] = p.rings[r0 + lane];
+                const int total = build_runs(p.hs, d, r0, lane, t);   /* ends with a wave sync */
+                walk_records<WRITE>(p, t, rl, r0, total, lane, c, sd, qn, sink);
+                wave_sync();
+            }
+            if (qn > 0)
+                drain_records(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, sink);
+        } else {
+            /* counting needs only each visit's beam pixel, which every
+             * certified form returns exactly: the cheaper single-precision
+             * form of the sky adjoint, without beam-row loads */
+            auto count = [&](const Pending &q) { atomicAdd(&cur[rec_band(R, q.bp)], 1u); };
+            for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
+                const int total = build_runs(p.hs, d, r0, lane, t);
+                walk_visits<false, false, decltype(count), false, false>(p, t, total, lane, c, sf,
+                                                                         sd, qn, count);
+                wave_sync();
+            }
+            if (qn > 0)
+                drain_exact<false>(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, count);
         }
-        if (qn > 0)
-            drain_records(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, sink);
     }
     if constexpr (!WRITE) {
         wave_sync();
