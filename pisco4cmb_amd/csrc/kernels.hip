@@ -321,11 +321,13 @@ __device__ inline void walk_visits(const PairParams &p, RunTable &t, int total, 
         if constexpr (LOAD_SKY) {
             /* forward: loads issued unconditionally (row 0 / pixel 0 when
              * the visit is dead), see above */
-            const float *row = p.beam + 8 * (long)(q.live ? bp : 0);
-            q.v0 = *(const float4 *)row;
-            q.v1 = *(const float2 *)(row + 4);
-            if constexpr (VT)
-                q.v2 = *(const float2 *)(row + 6);
+            if constexpr (LOAD_BEAM) {
+                const float *row = p.beam + 8 * (long)(q.live ? bp : 0);
+                q.v0 = *(const float4 *)row;
+                q.v1 = *(const float2 *)(row + 4);
+                if constexpr (VT)
+                    q.v2 = *(const float2 *)(row + 6);
+            }
             const int sp = q.live ? pix : 0;
             const float4 sv = p.sky4[sp];
             q.sI = sv.x;
@@ -2122,11 +2124,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjbeam_band(PairParams p, BeamBa
  *
  * The banded kernels above map each sample's annulus once per band of beam
  * rows and set up every (band, sample) pair.  Here each visit is mapped
- * twice, as in the forward, and the gradient is formed from 16-byte visit
+ * twice, once per pass, and the gradient is formed from 16-byte visit
  * records, per chunk of samples:
- *  1. k_beamrec<false> (one wave per sample, the forward's walk and mapping)
- *     counts the sample's visits inside the beam per band of `rows`
- *     consecutive beam rows;
+ *  1. k_beamrec<false> (one wave per sample, the sky adjoint's walk and
+ *     certified mapping) counts the sample's visits inside the beam per band
+ *     of `rows` consecutive beam rows;
  *  2. k_band_scan + k_scan_excl: per band, the offsets of the samples' records
  *     within the band; the bands' starts;
  *  3. k_beamrec<true> maps the visits again and writes each one's record
@@ -2191,131 +2193,11 @@ __device__ inline int rec_band(const RecParams &R, int bp)
     return b;
 }
 
-/* Counts (WRITE false) or writes (WRITE true) the live lanes' records;
- * called by the whole wave.  cur: per band, the count, or the next position
- * in band order. */
-template <bool WRITE>
-struct RecSink {
-    const RecParams &R;
-    unsigned *cur;
-    unsigned smp_key;
-    __device__ inline void operator()(bool live, int bp, float c2, float s2, float4 sv)
-    {
-        if (!live)
-            return;
-        const int b = rec_band(R, bp);
-        if constexpr (WRITE) {
-            const unsigned slot = atomicAdd(&cur[b], 1u);
-            if (slot >= (unsigned)R.nrec_cap)
-                return;   /* never: both passes count the same visits */
-            BeamRec r;
-            r.key = (unsigned)(bp - b * R.rows) | smp_key;
-            r.sI = sv.x;
-            r.X = sv.y * c2 + sv.z * s2;
-            r.Y = -sv.y * s2 + sv.z * c2;
-            R.brec[slot] = r;
-        } else {
-            atomicAdd(&cur[b], 1u);
-        }
-    }
-};
-
-/* Maps the deferred visits base .. base+cnt-1 (cnt <= 64, one per lane) with
- * the FP64-alpha form and hands them to the sink (whole wave). */
-template <typename Sink>
-__device__ inline void drain_records(const PairParams &p, RunTable &t, int base, int cnt, int lane,
-                                     const Centre &c, double c2psi, double s2psi, Sink &sink)
-{
-    wave_sync();
-    bool live = false;
-    int bp = -1;
-    float c2 = 0.f, s2 = 0.f;
-    float4 sv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < cnt) {
-        const int pix = t.qpix[base + lane], r = t.qring[base + lane];
-        const RingRec rr = p.rings[r];
-        bp = beam_pixel_f64(p.hb, p.fc64, c, c2psi, s2psi, ring_phi(p.hs, rr, r, pix), rr.cdp,
-                            rr.sdp, c2, s2);
-        live = (unsigned)bp < (unsigned)p.npix_beam;
-        if (live && Sink::kSky)
-            sv = p.sky4[pix];
-    }
-    sink(live, bp, c2, s2, sv);
-    __builtin_amdgcn_s_waitcnt(0);   /* as in drain_exact */
-    wave_sync();
-}
-
-/* The forward's walk (walk_visits with the FP64 form and its float alpha)
- * for records: no beam rows are read; with SKY, each visit's sky values are
- * loaded at the end of its iteration and used in the next one.  The pass's
- * ring records are read from rl (LDS; ring r at rl[r - r0]) instead of by
- * two global loads per visit: the write pass is bound by the TA issue of its
- * vector memory instructions (2.5% faster at the 5-degree, beam NSIDE 4096
- * shape). */
-template <bool SKY, typename Sink>
-__device__ inline void walk_records(const PairParams &p, RunTable &t, const RingRec *rl, int r0,
-                                    int total, int lane, const Centre &c, const SampleD &sd,
-                                    int &qn, Sink &sink)
-{
-    if (total <= 0)
-        return;
-    int s = 0;
-    int v = lane;
-    if (v < total)
-        while (v >= t.pref[s + 1])
-            s++;
-    RingRec rr = rl[t.ring[s] - r0];
-    __builtin_amdgcn_s_waitcnt(0);   /* as in walk_visits */
-    bool qlive = false;
-    int qbp = -1;
-    float qc2 = 0.f, qs2 = 0.f;
-    float4 qsv = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int v0 = 0; v0 < total; v0 += kWaveSize, v += kWaveSize) {
-        const bool act = v < total;
-        const int pix = t.pix0[s] + (act ? v - t.pref[s] : 0);
-        const int r = t.ring[s];
-        const RingRec cur = rr;
-        const int vn = v + kWaveSize;
-        if (vn < total)
-            while (vn >= t.pref[s + 1])
-                s++;
-        rr = rl[t.ring[s] - r0];
-        float c2, s2;
-        int bp = beam_pixel_f64<true>(p.hb, p.fc64, c, sd.c2psi, sd.s2psi,
-                                      ring_phi(p.hs, cur, r, pix), cur.cdp, cur.sdp, c2, s2);
-        bp = act ? bp : -1;
-        {
-            const unsigned long long m = __ballot(bp == kNeedExact);
-            if (m) {
-                if (bp == kNeedExact) {
-                    const int pos = qn + lanes_below(m);
-                    t.qpix[pos] = pix;
-                    t.qring[pos] = r;
-                }
-                qn += __popcll(m);
-            }
-        }
-        sink(qlive, qbp, qc2, qs2, qsv);
-        if (qn >= kWaveSize) {
-            drain_records(p, t, qn - kWaveSize, kWaveSize, lane, c, sd.c2psi, sd.s2psi, sink);
-            qn -= kWaveSize;
-        }
-        qlive = (unsigned)bp < (unsigned)p.npix_beam;
-        qbp = bp;
-        qc2 = c2;
-        qs2 = s2;
-        if constexpr (SKY)
-            qsv = p.sky4[qlive ? pix : 0];
-    }
-    sink(qlive, qbp, qc2, qs2, qsv);
-}
-
-template <bool WRITE>
-struct RecSinkT : RecSink<WRITE> {
-    static constexpr bool kSky = WRITE;
-};
-
-/* Steps 1 and 3: one wave per sample of the chunk [p.s_begin, p.s_end). */
+/* Steps 1 and 3: one wave per sample of the chunk [p.s_begin, p.s_end),
+ * the sky adjoint's walk and certified single-precision mapping (every
+ * visit's beam pixel is the exact one; cos/sin 2chi within 9.2e-7), with the
+ * deferred visits mapped by the FP64-alpha form.  The count pass loads no
+ * beam rows; the write pass loads each visit's sky values, pipelined. */
 template <bool WRITE>
 __global__ __launch_bounds__(WRITE ? kRecWriteWaves * kWaveSize : kBlock)
 __attribute__((amdgpu_waves_per_eu(4, 8)))
@@ -2323,13 +2205,11 @@ void k_beamrec(PairParams p, RecParams R)
 {
     constexpr int NW = WRITE ? kRecWriteWaves : kWavesPerBlock;   /* waves (samples) per block */
     __shared__ RunTable tables[NW];
-    __shared__ RingRec rls[NW][kWaveSize];   /* the ring records of each wave's pass */
     /* counts: one row per wave (sample); write cursors: one row per block */
     __shared__ unsigned curs[WRITE ? 1 : NW][kRecMaxBands];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWaveSize);
     const int lane = threadIdx.x % kWaveSize;
     RunTable &t = tables[wave];
-    RingRec *rl = rls[wave];
     const long j0 = xcd_remap(blockIdx.x, gridDim.x) * NW;
     const long j = j0 + wave;
     const long i = p.s_begin + j;
@@ -2356,7 +2236,6 @@ void k_beamrec(PairParams p, RecParams R)
             cur[b] = 0u;
         wave_sync();
     }
-    RecSinkT<WRITE> sink{{R, cur, (unsigned)j << kRecRowBits}};
     const float ga = unif(p.g_a ? p.g_a[i] : 0.f);
     const float gb = unif(p.g_b ? p.g_b[i] : 0.f);
     if (ga != 0.f || gb != 0.f) {   /* else it adds nothing (wave-uniform) */
@@ -2367,19 +2246,28 @@ void k_beamrec(PairParams p, RecParams R)
         sample_setup(p, i, p.pa[i], c, sf, sd, d);
         int qn = 0;
         if constexpr (WRITE) {
+            const unsigned smp_key = (unsigned)j << kRecRowBits;
+            auto write = [&](const Pending &q) {
+                const int b = rec_band(R, q.bp);
+                const unsigned slot = atomicAdd(&cur[b], 1u);
+                if (slot >= (unsigned)R.nrec_cap)
+                    return;   /* never: both passes count the same visits */
+                BeamRec r;
+                r.key = (unsigned)(q.bp - b * R.rows) | smp_key;
+                r.sI = q.sI;
+                r.X = q.sQ * q.c2 + q.sU * q.s2;
+                r.Y = -q.sQ * q.s2 + q.sU * q.c2;
+                R.brec[slot] = r;
+            };
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
-                if (r0 + lane <= d.r_last)
-                    rl[lane] = p.rings[r0 + lane];
-                const int total = build_runs(p.hs, d, r0, lane, t);   /* ends with a wave sync */
-                walk_records<WRITE>(p, t, rl, r0, total, lane, c, sd, qn, sink);
+                const int total = build_runs(p.hs, d, r0, lane, t);
+                walk_visits<true, false, decltype(write), false, false>(p, t, total, lane, c, sf,
+                                                                        sd, qn, write);
                 wave_sync();
             }
             if (qn > 0)
-                drain_records(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, sink);
+                drain_exact<true>(p, t, 0, qn, lane, c, sd.c2psi, sd.s2psi, write);
         } else {
-            /* counting needs only each visit's beam pixel, which every
-             * certified form returns exactly: the cheaper single-precision
-             * form of the sky adjoint, without beam-row loads */
             auto count = [&](const Pending &q) { atomicAdd(&cur[rec_band(R, q.bp)], 1u); };
             for (int r0 = d.r_first; r0 <= d.r_last; r0 += kWaveSize) {
                 const int total = build_runs(p.hs, d, r0, lane, t);
