@@ -1361,9 +1361,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjmap_lds(PairParams p, AdjLayou
     int *ctl = win_base + L.rw + 1;         /* 0 R0, 1 R1, 2 W, 3 slabs, 4.. slab starts */
     int *slab = ctl + 4;                    /* first window ring of each slab, then nring */
     float *acc = (float *)(ctl + kAdjCtl);
-    /* deterministic mode: int64 windows, 8-byte aligned after ctl */
-    unsigned long long *acc64 =
-        (unsigned long long *)(((unsigned long)(ctl + kAdjCtl) + 7ul) & ~7ul);
+    /* deterministic mode: int64 windows, 8-byte aligned after ctl.  Formed
+     * by indexing smem (16-byte aligned), not through an integer: a pointer
+     * that passes through an integer loses its LDS address space, and its
+     * atomics compile to flat instead of ds_add_u64 (which measured the
+     * same here: the window's adds are off the critical path). */
+    const size_t acc_off = (size_t)((unsigned char *)(ctl + kAdjCtl) - smem);
+    unsigned long long *acc64 = (unsigned long long *)(smem + ((acc_off + 7) & ~(size_t)7));
     const double fxs = DET ? *L.fx_scale : 0.0;
 
     const int tid = threadIdx.x;
