@@ -5,7 +5,8 @@ in band order and summed band by band in LDS as 64-bit fixed-point integers
 runs in chunks of 256 samples), the annulus walks over LDS bands
 (adjbeam_records = 0; 8 or 64 lanes per sample, small and large work items)
 and global atomics (adjbeam_bands = 0).  Shapes: the headline beam (113
-bands of records, 81 annulus bands), a cap reaching the equatorial rings of
+bands of records, 81 annulus bands; a sample count that leaves the record
+form's last write block of 8 samples partial), a cap reaching the equatorial rings of
 a coarse beam map (bands of 300 rows: whole and partial rings), samples near
 the poles, small discs, beams of one ring and a last band of one row; the
 record form's bits repeat from call to call; and the dot product with the
@@ -76,7 +77,9 @@ def test_headline_beam_shape(oracle, kind):
     ns = 512
     sky = inputs.iid_sky(ns, seed=82)
     beam = inputs.make_polbeam(kind, 4096, 5.0, 0.33, 0.05, 0.01, nside_sky=ns)
-    ra, dec, pa = inputs.ces_scan(400, t0=1300.0)
+    # 397 samples: the record form's last write block (8 samples) is partial,
+    # and its band-sum parts are cut at multiples of 8 samples
+    ra, dec, pa = inputs.ces_scan(397, t0=1300.0)
     conv, ga, gb, res = check_against_oracle(oracle, ns, sky, beam, ra, dec, pa,
                                              chunks=(1024, 7))
     # the record form sums integers: the same bits on every call
