@@ -426,7 +426,10 @@ int pisco_dev_match_mode(pisco_ctx *ctx, void *stream, int mode, int nside_sky,
                          const float *dec, const float *pa, const long *offsets,
                          const int *pixels, int *bpix, double *chi);
 /* Elementwise FP64 primitives of the index path on the device (op: 0 sin,
- * 1 cos, 2 acos, 3 atan2(x,y), 4 sqrt, 5 x/y, 6 fmodulo(x,y), 7 sincos). */
+ * 1 cos, 2 acos, 3 atan2(x,y), 4 sqrt, 5 x/y, 6 fmodulo(x,y), 7 sincos;
+ * 8 the fixed-point conversion of the integer sums: rint((float)x * y) for
+ * a power of two y and |x y| < 2^51, as an int64 stored in the output's
+ * 8 bytes). */
 int pisco_dev_libm(pisco_ctx *ctx, void *stream, int op, long n,
                    const double *x, const double *y, double *out);
 

@@ -1278,7 +1278,7 @@ int pisco_dev_match_mode(pisco_ctx *c, void *stream, int mode, int nside_sky, in
 int pisco_dev_libm(pisco_ctx *c, void *stream, int op, long n, const double *x,
                    const double *y, double *out)
 {
-    if (!c || op < 0 || op > 7 || n < 0)
+    if (!c || op < 0 || op > 8 || n < 0)
         return fail(PISCO_ERR_INVALID, "bad argument");
     if (n == 0)
         return PISCO_OK;

@@ -2749,6 +2749,7 @@ __global__ void k_libm(int op, long n, const double *x, const double *y, double 
         case 5: r = a / b; break;
         case 6: r = fmodulo(a, b); break;
         case 7: { double s, c; pl_sincos(a, &s, &c); r = s + 2.0 * c; } break;
+        case 8: r = __longlong_as_double((long long)fix64((float)a, b)); break;
         default: r = 0.0;
         }
         o[i] = r;

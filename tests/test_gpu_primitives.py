@@ -51,6 +51,31 @@ def test_sqrt_div_fmod_correctly_rounded(gpu):
         assert np.array_equal(got.view(np.int64), ref.view(np.int64)), m
 
 
+def test_fixed_point_conversion_is_rint(gpu):
+    """The integer sums' conversion (one fma with 1.5 * 2^52 and an integer
+    subtract) equals rint(v * s) as int64, ties to even, for float v and
+    power-of-two scales s with |v s| < 2^50 (the bound the scales keep)."""
+    rng = np.random.default_rng(13)
+    n = 200000
+    k = rng.integers(-20, 60, n)
+    s = np.ldexp(1.0, k)
+    v = (rng.normal(size=n) * np.ldexp(1.0, rng.integers(-30, 10, n))).astype(np.float32)
+    # keep |v s| < 2^50
+    v = np.where(np.abs(v.astype(np.float64) * s) < 2.0 ** 50, v, np.float32(0.75))
+    s = np.where(np.abs(v.astype(np.float64) * s) < 2.0 ** 50, s, 1.0)
+    # exact ties of both signs, zeros and the ends of the range
+    m = rng.integers(-2 ** 20, 2 ** 20, 2000)
+    tv = ((m + 0.5) / 2.0).astype(np.float32)
+    v = np.concatenate([v, tv, -tv, np.float32([0.0, -0.0, 1.0, -1.0, 0.5, -0.5, 1.5, -1.5])])
+    s = np.concatenate([s, np.full(2 * m.size, 2.0), np.ones(8)])
+    big = np.float32([2.0 ** 23 - 1, -(2.0 ** 23 - 1)])
+    v = np.concatenate([v, big])
+    s = np.concatenate([s, np.full(2, 2.0 ** 26)])
+    got = gpu.libm(8, v.astype(np.float64), s).view(np.int64)
+    ref = np.rint(v.astype(np.float64) * s).astype(np.int64)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("nside", [1, 2, 4, 8, 16, 64, 128, 256, 512, 1024, 2048])
 def test_pix2ang_ang2pix_all_pixels(gpu, oracle, nside):
     npix = 12 * nside * nside
