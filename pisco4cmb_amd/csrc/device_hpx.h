@@ -360,16 +360,17 @@ __host__ __device__ inline int disc_ring(const Hpx &h, const Disc &d, int r, int
         ip_lo -= nr;
         ip_hi -= nr;
     }
-    if (ip_lo < 0) {
-        int n = 0;
+    if (ip_lo < 0) {   /* (constant indices: a runtime index would put the arrays in scratch) */
         if (ip_hi + 1 > 0) {
-            pix0[n] = ipix1;
-            len[n] = ip_hi + 1;
-            n++;
+            pix0[0] = ipix1;
+            len[0] = ip_hi + 1;
+            pix0[1] = ipix1 + ip_lo + nr;
+            len[1] = -ip_lo;
+            return 2;
         }
-        pix0[n] = ipix1 + ip_lo + nr;
-        len[n] = -ip_lo;
-        return n + 1;
+        pix0[0] = ipix1 + ip_lo + nr;
+        len[0] = -ip_lo;
+        return 1;
     }
     pix0[0] = ipix1 + ip_lo;
     len[0] = ip_hi - ip_lo + 1;

@@ -105,11 +105,14 @@ __device__ inline int build_runs(const Hpx &hs, const Disc &d, int r0, int lane,
     int tot_in = wave_incl_scan(tot, lane);
     int cbase = cnt_in - cnt;
     int off = tot_in - tot;
-    for (int k = 0; k < ns; k++) {
-        t.pix0[cbase + k] = p0[k];
-        t.ring[cbase + k] = r;
-        t.pref[cbase + k] = off;
-        off += ln[k];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {   /* constant indices keep p0, ln in registers */
+        if (k < ns) {
+            t.pix0[cbase + k] = p0[k];
+            t.ring[cbase + k] = r;
+            t.pref[cbase + k] = off;
+            off += ln[k];
+        }
     }
     int nrun = __shfl(cnt_in, kWaveSize - 1, kWaveSize);
     int total = __shfl(tot_in, kWaveSize - 1, kWaveSize);
@@ -517,11 +520,14 @@ __device__ inline int build_sub(const Hpx &hs, const Disc &d, int nrings, int ps
     const int tot_in = seg_incl_scan<SUB>(tot, sl);
     const int cbase = cnt_in - ns;
     int off = tot_in - tot;
-    for (int k = 0; k < ns; k++) {
-        t.pix0[cbase + k] = p0[k];
-        t.ring[cbase + k] = r;
-        t.pref[cbase + k] = off;
-        off += ln[k];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {   /* constant indices keep p0, ln in registers */
+        if (k < ns) {
+            t.pix0[cbase + k] = p0[k];
+            t.ring[cbase + k] = r;
+            t.pref[cbase + k] = off;
+            off += ln[k];
+        }
     }
     const int nrun = __shfl(cnt_in, SUB - 1, SUB);
     const int total = __shfl(tot_in, SUB - 1, SUB);
@@ -1002,8 +1008,9 @@ __device__ inline bool batch_exact_visit(const BatchParams &p, const BatchPair &
     int p0[2] = {0, 0}, ln[2] = {0, 0};
     const int nrun = disc_ring(p.hs, dk, r, p0, ln);
     bool in = false;
-    for (int k = 0; k < nrun; k++)
-        in = in || ((unsigned)(pix - p0[k]) < (unsigned)ln[k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        in = in || (k < nrun && (unsigned)(pix - p0[k]) < (unsigned)ln[k]);
     if (!in)
         return false;
     const RingRec rr = p.rings[r];
@@ -1826,11 +1833,14 @@ __device__ inline int build_band_runs(const Hpx &hs, const Disc &dout, const Dis
     const int tot_in = wave_incl_scan(tot, lane);
     const int cbase = cnt_in - ns;
     int off = tot_in - tot;
-    for (int k = 0; k < ns; k++) {
-        t.pix0[cbase + k] = p0[k];
-        t.ring[cbase + k] = r;
-        t.pref[cbase + k] = off;
-        off += ln[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {   /* constant indices keep p0, ln in registers */
+        if (k < ns) {
+            t.pix0[cbase + k] = p0[k];
+            t.ring[cbase + k] = r;
+            t.pref[cbase + k] = off;
+            off += ln[k];
+        }
     }
     const int nrun = __shfl(cnt_in, kWaveSize - 1, kWaveSize);
     const int total = __shfl(tot_in, kWaveSize - 1, kWaveSize);
@@ -1915,11 +1925,14 @@ __device__ inline int build_band_sub(const Hpx &hs, const Disc &dout, const Disc
     const int tot_in = seg_incl_scan<SUB>(tot, sl);
     const int cbase = cnt_in - ns;
     int off = tot_in - tot;
-    for (int k = 0; k < ns; k++) {
-        t.pix0[cbase + k] = p0[k];
-        t.ring[cbase + k] = r;
-        t.pref[cbase + k] = off;
-        off += ln[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {   /* constant indices keep p0, ln in registers */
+        if (k < ns) {
+            t.pix0[cbase + k] = p0[k];
+            t.ring[cbase + k] = r;
+            t.pref[cbase + k] = off;
+            off += ln[k];
+        }
     }
     const int nrun = __shfl(cnt_in, SUB - 1, SUB);
     const int total = __shfl(tot_in, SUB - 1, SUB);
@@ -2656,13 +2669,16 @@ __global__ void k_disc(Hpx h, long n, const float *ra, const float *dec, double 
     long k = pixels ? offsets[i] : 0;
     int cnt = 0;
     for (int r = d.r_first; r <= d.r_last; r++) {
-        int p0[2], ln[2];
+        int p0[2] = {0, 0}, ln[2] = {0, 0};
         int ns = disc_ring(h, d, r, p0, ln);
-        for (int j = 0; j < ns; j++) {
-            if (pixels)
-                for (int q = 0; q < ln[j]; q++)
-                    pixels[k + cnt + q] = p0[j] + q;
-            cnt += ln[j];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            if (j < ns) {
+                if (pixels)
+                    for (int q = 0; q < ln[j]; q++)
+                        pixels[k + cnt + q] = p0[j] + q;
+                cnt += ln[j];
+            }
         }
     }
     if (counts)
